@@ -1,0 +1,252 @@
+"""ctypes view of the C-ABI declared in include/nrg_chem.h.
+
+The structures here are byte-for-byte the C ones; ``load_library`` opens the
+in-tree HIP build ``nrg_amd/libnrg_chem.so`` and fails loudly when it is
+missing -- there is no CPU fallback behind this interface.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libnrg_chem.so")
+
+# status codes (nrg_chem.h)
+NRG_CHEM_OK = 0
+NRG_CHEM_E_ARG = 1
+NRG_CHEM_E_MECH = 2
+NRG_CHEM_E_HIP = 3
+NRG_CHEM_E_NO_DEVICE = 4
+NRG_CHEM_E_STEP_ZERO = 41
+NRG_CHEM_E_REPEATED_FAIL = 42
+
+NRG_CELL_OK = 0
+NRG_CELL_GATED = 1
+NRG_CELL_MXSTEP = 2
+NRG_CELL_STEP_ZERO = 3
+NRG_CELL_REPEATED_FAIL = 4
+NRG_CELL_SINGULAR = 5
+
+NRG_JAC_FD_SLATEC = 0
+NRG_JAC_ANALYTIC = 1
+
+
+class NrgChemMech(C.Structure):
+    _fields_ = [
+        ("species_number", C.c_int32),
+        ("reactions_number", C.c_int32),
+        ("reactions_type", C.POINTER(C.c_int32)),
+        ("chemical_coeffs", C.POINTER(C.c_int32)),
+        ("A", C.POINTER(C.c_double)),
+        ("beta", C.POINTER(C.c_double)),
+        ("E_act", C.POINTER(C.c_double)),
+        ("A_low", C.POINTER(C.c_double)),
+        ("beta_low", C.POINTER(C.c_double)),
+        ("E_act_low", C.POINTER(C.c_double)),
+        ("Troe_coeffs", C.POINTER(C.c_double)),
+        ("enhanced_efficiencies", C.POINTER(C.c_double)),
+        ("a_coeffs", C.POINTER(C.c_double)),
+        ("molar_masses", C.POINTER(C.c_double)),
+    ]
+
+
+class NrgChemOpts(C.Structure):
+    _fields_ = [
+        ("eps", C.c_double),
+        ("ewt", C.c_double),
+        ("hmax", C.c_double),
+        ("t_gate", C.c_double),
+        ("mxstep", C.c_int32),
+        ("mxord", C.c_int32),
+        ("jacobian", C.c_int32),
+        ("device", C.c_int32),
+        ("block_size", C.c_int32),
+        ("reserved", C.c_int32 * 7),
+    ]
+
+
+class NrgGrid(C.Structure):
+    _fields_ = [
+        ("lo", C.c_int32 * 3),
+        ("hi", C.c_int32 * 3),
+        ("inner_lo", C.c_int32 * 3),
+        ("inner_hi", C.c_int32 * 3),
+    ]
+
+
+class NrgChemStats(C.Structure):
+    _fields_ = [
+        ("cells_inner", C.c_int64),
+        ("cells_integrated", C.c_int64),
+        ("nstep_sum", C.c_int64),
+        ("nfe_sum", C.c_int64),
+        ("nje_sum", C.c_int64),
+        ("ncorr_sum", C.c_int64),
+        ("natt_sum", C.c_int64),
+        ("nstep_max", C.c_int32),
+        ("nfe_max", C.c_int32),
+        ("status_count", C.c_int64 * 8),
+        ("kernel_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {
+            "cells_inner": self.cells_inner,
+            "cells_integrated": self.cells_integrated,
+            "nstep_sum": self.nstep_sum,
+            "nfe_sum": self.nfe_sum,
+            "nje_sum": self.nje_sum,
+            "ncorr_sum": self.ncorr_sum,
+            "natt_sum": self.natt_sum,
+            "nstep_max": self.nstep_max,
+            "nfe_max": self.nfe_max,
+            "status_count": list(self.status_count),
+            "kernel_ms": self.kernel_ms,
+        }
+
+
+def default_opts(**kw):
+    """Reference literals (chemical_kinetics_solver.f90:270-290), see nrg_chem_default_opts."""
+    o = NrgChemOpts()
+    o.eps = 1.0e-9
+    o.ewt = 1.0e-4
+    o.hmax = float(np.float32(1.0e-7))
+    o.t_gate = 305.0
+    o.mxstep = 10000
+    o.mxord = 5
+    o.jacobian = NRG_JAC_FD_SLATEC
+    o.device = 0
+    o.block_size = 0
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def grid_for_shape(shape, ghost=1):
+    """Grid of NRG fields with `ghost` layers in active dimensions (extent > 1).
+
+    Mirrors computational_domain bounds (0:n+1 in active dims, 1:1 otherwise).
+    """
+    g = NrgGrid()
+    for d in range(3):
+        n = shape[d] if d < len(shape) else 1
+        gh = ghost if n > 1 else 0
+        g.lo[d] = 1 - gh
+        g.hi[d] = n + gh
+        g.inner_lo[d] = 1
+        g.inner_hi[d] = n
+    return g
+
+
+def flat_grid(ncells):
+    """A 1-D list of cells with no ghosts: offset == cell index."""
+    g = NrgGrid()
+    g.lo[0], g.hi[0], g.inner_lo[0], g.inner_hi[0] = 1, ncells, 1, ncells
+    for d in (1, 2):
+        g.lo[d] = g.hi[d] = g.inner_lo[d] = g.inner_hi[d] = 1
+    return g
+
+
+class MechStruct:
+    """Owns contiguous copies of a Mechanism's arrays and the C struct pointing at them."""
+
+    def __init__(self, mech):
+        self.mech = mech
+        self._keep = {}
+
+        def dptr(name, arr, dtype=np.float64):
+            a = np.ascontiguousarray(arr, dtype=dtype)
+            self._keep[name] = a
+            ctype = C.c_double if dtype == np.float64 else C.c_int32
+            return a.ctypes.data_as(C.POINTER(ctype))
+
+        s = NrgChemMech()
+        s.species_number = mech.species_number
+        s.reactions_number = mech.reactions_number
+        s.reactions_type = dptr("rt", mech.reactions_type, np.int32)
+        s.chemical_coeffs = dptr("cc", mech.chemical_coeffs, np.int32)
+        s.A = dptr("A", mech.A)
+        s.beta = dptr("beta", mech.beta)
+        s.E_act = dptr("E", mech.E_act)
+        s.A_low = dptr("Al", mech.A_low)
+        s.beta_low = dptr("bl", mech.beta_low)
+        s.E_act_low = dptr("El", mech.E_act_low)
+        s.Troe_coeffs = dptr("troe", mech.Troe_coeffs)
+        s.enhanced_efficiencies = dptr("eff", mech.enhanced_efficiencies)
+        s.a_coeffs = dptr("a", mech.a_coeffs)
+        s.molar_masses = dptr("W", mech.molar_masses)
+        self.struct = s
+
+    def ref(self):
+        return C.byref(self.struct)
+
+
+def ptr_array(ptrs):
+    """C array of `double*` (device or host addresses given as ints)."""
+    arr = (C.c_void_p * len(ptrs))(*[C.c_void_p(int(p)) for p in ptrs])
+    return arr
+
+
+class NrgChemError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Open libnrg_chem.so (the HIP build).  Raises if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(
+            "libnrg_chem.so is not built (%s); run __graft_entry__.build() "
+            "or `make -C nrg_amd/csrc`" % p)
+    lib = C.CDLL(p)
+    vp = C.c_void_p
+    lib.nrg_chem_default_opts.argtypes = [C.POINTER(NrgChemOpts)]
+    lib.nrg_chem_default_opts.restype = None
+    lib.nrg_chem_strerror.argtypes = [C.c_int]
+    lib.nrg_chem_strerror.restype = C.c_char_p
+    lib.nrg_chem_last_error.argtypes = []
+    lib.nrg_chem_last_error.restype = C.c_char_p
+    lib.nrg_chem_abi_version.argtypes = []
+    lib.nrg_chem_abi_version.restype = C.c_int
+    lib.nrg_chem_create.argtypes = [C.POINTER(NrgChemMech), C.POINTER(NrgChemOpts), C.POINTER(vp)]
+    lib.nrg_chem_create.restype = C.c_int
+    lib.nrg_chem_destroy.argtypes = [vp]
+    lib.nrg_chem_destroy.restype = None
+    step_args = [vp, C.POINTER(NrgGrid), C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                 C.POINTER(NrgChemStats)]
+    lib.nrg_chem_step_dev.argtypes = step_args
+    lib.nrg_chem_step_dev.restype = C.c_int
+    lib.nrg_chem_step.argtypes = step_args
+    lib.nrg_chem_step.restype = C.c_int
+    lib.nrg_chem_closure_dev.argtypes = [vp, C.POINTER(NrgGrid), C.c_double, vp, vp, C.c_int32,
+                                         vp, vp, vp, vp, vp, vp, vp]
+    lib.nrg_chem_closure_dev.restype = C.c_int
+    lib.nrg_state_equation_dev.argtypes = [vp, C.POINTER(NrgGrid), vp, vp, C.c_int32, vp, vp, vp,
+                                           vp, vp]
+    lib.nrg_state_equation_dev.restype = C.c_int
+    lib.nrg_energy_from_temperature_dev.argtypes = [vp, C.POINTER(NrgGrid), vp, vp, vp, C.c_int32,
+                                                    vp]
+    lib.nrg_energy_from_temperature_dev.restype = C.c_int
+    lib.nrg_chem_stream.argtypes = [vp]
+    lib.nrg_chem_stream.restype = vp
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(lib, status):
+    if status != NRG_CHEM_OK:
+        detail = lib.nrg_chem_last_error().decode(errors="replace")
+        raise NrgChemError(status, "%s: %s" % (lib.nrg_chem_strerror(status).decode(), detail))
+    return status

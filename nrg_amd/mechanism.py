@@ -1,0 +1,532 @@
+"""Mechanism and thermochemistry tables, read the way NRG reads them.
+
+NRG keeps a reaction mechanism in ``chemical_properties`` and NASA-7 / molar-mass
+data in ``thermophysical_properties``; the chemistry step only ever sees the
+arrays those two types hold after ``set_properties``.  This module produces the
+same arrays from the same text files, so the C-ABI receives exactly what NRG's
+Fortran shim would hand over.
+
+Reference behaviour followed (file:line in yakovenko-ivan/NRG):
+
+* species list: ``package_library/src/chemical_properties_class.f90:685-768``
+  (names after ``SPECIES`` up to ``END``; index S+1 is the pseudo-species ``M``);
+* reaction count / typing / coefficients:
+  ``chemical_properties_class.f90:222-451`` and the left/right parser
+  ``:453-635`` (``'<=>'``, ``'=>'``, ``'='``; ``+M`` -> type 1 (6 if
+  irreversible); ``(+M)`` with ``LOW`` -> 2 (7), with ``TROE`` -> 3 (7);
+  an unknown species drops the whole reaction; the "enhanced efficiency"
+  flag is sticky once set);
+* unit scaling: ``chemical_properties_class.f90:162-180`` (A x1e-6 / x1e-12 by
+  the number of left terms, A_low by that number + 1, E_act by the energy unit);
+* NASA-7 coefficients x R_u: ``package_library/src/thermophysical_properties_class.f90:161-169``
+  with the tokenizer of ``get_a_coeffs`` ``:1226-1311``;
+* molar masses: ``thermophysical_properties_class.f90:1313-1341``.
+
+Array layouts are the Fortran ones (column-major), stored here as C-order numpy
+arrays with the axes reversed, so ``arr.ctypes.data`` is bit-for-bit the memory
+NRG holds:
+
+===================  ===================  ======================
+Fortran array        numpy shape           element (Fortran idx)
+===================  ===================  ======================
+chemical_coeffs(4,R,2)  (2, R, 4)          [side, r, c]
+Troe_coeffs(R,4)        (4, R)             [k, r]
+enhanced_efficiencies(R,S)  (S, R)         [s, r]
+a_coeffs(7,2,S)         (S, 2, 7)          [s, set, i]
+===================  ===================  ======================
+"""
+from __future__ import annotations
+
+import os
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Physical constants exactly as package_library/src/global_data.f90:35-44.
+R_GAS_J = 8.31446261815324
+R_GAS_CAL = 1.98720425864083
+T_REF = 298.15
+P_ATM = 101325.0
+TABLES_TEMPERATURE_CEILING = 6000.0
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+# reaction types (chemical_properties_class.f90:339-406, chemical_kinetics_solver.f90:412-485)
+RT_ELEMENTARY = 0
+RT_THIRD_BODY = 1
+RT_LINDEMANN = 2
+RT_TROE = 3
+RT_IRREVERSIBLE = 5
+RT_IRREVERSIBLE_THIRD_BODY = 6
+RT_IRREVERSIBLE_FALLOFF = 7
+
+
+class MechanismError(ValueError):
+    pass
+
+
+def _fortran_real(tok: str) -> float:
+    """Fortran list-directed real: accepts '4.8749+04' (exponent letter omitted)."""
+    t = tok.strip().rstrip(",")
+    m = re.fullmatch(r"([+-]?(?:\d+\.?\d*|\.\d+))([+-]\d+)", t)
+    if m:
+        t = m.group(1) + "E" + m.group(2)
+    t = t.replace("D", "E").replace("d", "e")
+    return float(t)
+
+
+def _list_directed_items(line: str):
+    """Split one record the way Fortran list-directed input does.
+
+    Blank/comma separated; a quoted string is one item; an unquoted '/'
+    terminates the record.
+    """
+    items = []
+    i, n = 0, len(line)
+    while i < n:
+        c = line[i]
+        if c in " \t,":
+            i += 1
+            continue
+        if c == "/":
+            break
+        if c in "'\"":
+            q = c
+            j = i + 1
+            buf = []
+            while j < n:
+                if line[j] == q:
+                    if j + 1 < n and line[j + 1] == q:
+                        buf.append(q)
+                        j += 2
+                        continue
+                    break
+                buf.append(line[j])
+                j += 1
+            items.append("".join(buf))
+            i = j + 1
+            continue
+        j = i
+        while j < n and line[j] not in " \t,/":
+            j += 1
+        items.append(line[i:j])
+        i = j
+    return items
+
+
+class _Records:
+    """Sequential record reader with Fortran-like ``read(*)``/``backspace``."""
+
+    def __init__(self, lines):
+        self.lines = [ln.rstrip("\r\n") for ln in lines]
+        self.pos = 0
+
+    def read_items(self, nitems=1):
+        """List-directed read of ``nitems`` values; blank records are skipped."""
+        got = []
+        while len(got) < nitems:
+            if self.pos >= len(self.lines):
+                raise EOFError
+            items = _list_directed_items(self.lines[self.pos])
+            self.pos += 1
+            got.extend(items[: nitems - len(got)])
+        return got
+
+    def read_raw(self):
+        if self.pos >= len(self.lines):
+            raise EOFError
+        s = self.lines[self.pos]
+        self.pos += 1
+        return s
+
+    def backspace(self):
+        self.pos -= 1
+
+
+def _read_lines(path):
+    with open(path, "rb") as f:
+        return f.read().decode("latin-1").splitlines()
+
+
+def _species_names(lines):
+    """SPECIES block: blank-separated names until END (chemical_properties_class.f90:728-768)."""
+    names = []
+    k = 0
+    while k < len(lines) and _list_directed_items(lines[k])[:1] != ["SPECIES"]:
+        k += 1
+    k += 1
+    while k < len(lines):
+        for tok in lines[k].split():
+            if tok == "END":
+                return names
+            names.append(tok)
+        k += 1
+    raise MechanismError("SPECIES block has no END")
+
+
+def _parse_side(side: str, index):
+    """Left or right side of one reaction (chemical_properties_class.f90:488-634).
+
+    Returns (count, [idx1, idx2, idx3], unknown_flag); indices are 1-based,
+    M = S+1, 0 = unused.
+    """
+    sp = [0, 0, 0]
+    unknown = False
+    pos = side.find("+")
+    if pos < 0:
+        if side[:1] == "2":
+            sp[0] = index(side[1:])
+            sp[1] = sp[0]
+            count = 2
+        else:
+            sp[0] = index(side)
+            count = 1
+        unknown |= sp[0] == 0
+        return count, sp, unknown
+    first = side[:pos]
+    doubled = False
+    if first[:1] == "2":
+        sp[0] = index(first[1:])
+        sp[1] = sp[0]
+        count = 2
+        doubled = True
+    else:
+        sp[0] = index(first)
+        count = 1
+    unknown |= sp[0] == 0
+    rest = side[pos + 1:]
+    pos = rest.find("+")
+    if pos < 0:
+        if rest[:1] == "2":
+            sp[1] = index(rest[1:])
+            sp[2] = sp[1]
+            count = 3
+            unknown |= sp[1] == 0
+        elif doubled:
+            sp[2] = index(rest)
+            count = 3
+            unknown |= sp[2] == 0
+        else:
+            sp[1] = index(rest)
+            count = 2
+            unknown |= sp[1] == 0
+    else:
+        sp[1] = index(rest[:pos])
+        unknown |= sp[1] == 0
+        sp[2] = index(rest[pos + 1:])
+        unknown |= sp[2] == 0
+        count = 3
+    return count, sp, unknown
+
+
+def _split_reaction(string: str):
+    """Return (r_type, left, right) with r_type 1 '=', 2 '=>', 3 '<=>' (:327-374)."""
+    r_type = 0
+    if "=" in string:
+        r_type = 1
+    if "=>" in string:
+        r_type = 2
+    if "<=" in string:
+        r_type = 3
+    if r_type == 0:
+        return 0, None, None
+    if r_type == 1:
+        p = string.find("=")
+        return 1, string[:p], string[p + 1:]
+    if r_type == 2:
+        p = string.find("=>")
+        return 2, string[:p], string[p + 2:]
+    p = string.find("<=")
+    return 3, string[:p], string[p + 3:]
+
+
+@dataclass
+class Mechanism:
+    species_names: list
+    reactions_type: np.ndarray          # int32 [R]
+    chemical_coeffs: np.ndarray         # int32 (2, R, 4)  == Fortran (4,R,2)
+    A: np.ndarray
+    beta: np.ndarray
+    E_act: np.ndarray
+    A_low: np.ndarray
+    beta_low: np.ndarray
+    E_act_low: np.ndarray
+    Troe_coeffs: np.ndarray             # (4, R)  == Fortran (R,4)
+    enhanced_efficiencies: np.ndarray   # (S, R)  == Fortran (R,S)
+    a_coeffs: np.ndarray                # (S, 2, 7) == Fortran (7,2,S), already x R_u
+    molar_masses: np.ndarray            # [S] kg/mol
+    reaction_strings: list = field(default_factory=list)
+
+    @property
+    def species_number(self):
+        return len(self.species_names)
+
+    @property
+    def reactions_number(self):
+        return int(self.reactions_type.shape[0])
+
+    def species_index(self, name):
+        """1-based index as get_chemical_specie_index (chemical_properties_class.f90:637-658)."""
+        for i, n in enumerate(self.species_names + ["M"]):
+            if n == name:
+                return i + 1
+        return 0
+
+
+def parse_mechanism(mech_path, default_enhanced_efficiencies=1.0, E_act_units="cal.mol"):
+    """Parse a CHEMKIN-like mechanism file as chemical_properties%set_properties does."""
+    lines = _read_lines(mech_path)
+    species = _species_names(lines)
+    S = len(species)
+    names_m = species + ["M"]
+
+    def index(name):
+        name = name.strip()
+        for i, n in enumerate(names_m):
+            if n == name:
+                return i + 1
+        return 0
+
+    rec = _Records(lines)
+    # skip to REACTIONS
+    while True:
+        if rec.read_items(1)[0] == "REACTIONS":
+            break
+
+    rtypes, coeffs, A, beta, E = [], [], [], [], []
+    A_low, beta_low, E_low, troe, effs, rstrings = [], [], [], [], [], []
+    enhanced_efficiency = False
+    while True:
+        try:
+            string = rec.read_items(1)[0]
+        except EOFError:
+            break
+        original = string
+        third_body_position = string.find("+M")
+        fall_off = string.find("(+M)")
+        if fall_off >= 0:
+            trimmed = string.rstrip()
+            string = string[:fall_off] + string[fall_off + 4:len(trimmed) - 4]
+            enhanced_efficiency = True
+        r_type, left, right = _split_reaction(string)
+        if r_type == 0:
+            continue
+        nl, spl, unk_l = _parse_side(left.strip(), index)
+        nr, spr, unk_r = _parse_side(right.strip(), index)
+        if unk_l or unk_r:
+            continue
+        if r_type == 2:
+            rt = RT_IRREVERSIBLE_THIRD_BODY if third_body_position >= 0 else RT_IRREVERSIBLE
+        else:
+            rt = RT_THIRD_BODY if third_body_position >= 0 else RT_ELEMENTARY
+        if third_body_position >= 0:
+            enhanced_efficiency = True
+        eff = np.full(S, float(default_enhanced_efficiencies))
+        tr = [0.0, 0.0, 0.0, 0.0]
+        a_lo = b_lo = e_lo = 0.0
+        rec.backspace()
+        items = rec.read_items(4)
+        a, b, e = (_fortran_real(x) for x in items[1:4])
+        if fall_off >= 0:
+            tok = rec.read_items(1)[0]
+            if tok == "LOW":
+                rt = RT_IRREVERSIBLE_FALLOFF if r_type == 2 else RT_LINDEMANN
+                rec.backspace()
+                items = rec.read_items(4)
+                a_lo, b_lo, e_lo = (_fortran_real(x) for x in items[1:4])
+            tok = rec.read_items(1)[0]
+            if tok == "TROE":
+                rt = RT_IRREVERSIBLE_FALLOFF if r_type == 2 else RT_TROE
+                rec.backspace()
+                items = rec.read_items(5)
+                tr = [_fortran_real(x) for x in items[1:5]]
+            else:
+                rec.backspace()
+        # efficiency record (chemical_properties_class.f90:421-445)
+        try:
+            tok = rec.read_items(1)[0]
+        except EOFError:
+            tok = ""
+        if "/" in tok and enhanced_efficiency:
+            rest = tok
+            while rest.strip():
+                p1 = rest.find("/")
+                sname = rest[:p1].strip()
+                rest = rest[p1 + 1:]
+                p2 = rest.find("/")
+                sidx = index(sname)
+                if 0 < sidx <= S:
+                    eff[sidx - 1] = _fortran_real(rest[:p2])
+                rest = rest[p2 + 1:]
+        else:
+            rec.backspace()
+        rtypes.append(rt)
+        coeffs.append(([nl] + spl, [nr] + spr))
+        A.append(a)
+        beta.append(b)
+        E.append(e)
+        A_low.append(a_lo)
+        beta_low.append(b_lo)
+        E_low.append(e_lo)
+        troe.append(tr)
+        effs.append(eff)
+        rstrings.append(original)
+
+    R = len(rtypes)
+    cc = np.zeros((2, R, 4), dtype=np.int32)
+    for r, (lft, rgt) in enumerate(coeffs):
+        cc[0, r, :] = lft
+        cc[1, r, :] = rgt
+    A = np.array(A, dtype=np.float64)
+    A_low = np.array(A_low, dtype=np.float64)
+    E = np.array(E, dtype=np.float64)
+    E_low = np.array(E_low, dtype=np.float64)
+    if E_act_units == "J.mol":
+        coef = 1.0
+    elif E_act_units == "kJ.mol":
+        coef = 1000.0
+    elif E_act_units == "cal.mol":
+        coef = R_GAS_J / R_GAS_CAL
+    elif E_act_units == "kcal.mol":
+        coef = R_GAS_J / R_GAS_CAL * 1000.0
+    else:
+        raise MechanismError("unknown E_act_units %r" % E_act_units)
+    for r in range(R):
+        if cc[0, r, 0] == 2:
+            A[r] = A[r] * 1.0e-06
+        if cc[0, r, 0] == 3:
+            A[r] = A[r] * 1.0e-12
+        if cc[0, r, 0] + 1 == 2:
+            A_low[r] = A_low[r] * 1.0e-06
+        if cc[0, r, 0] + 1 == 3:
+            A_low[r] = A_low[r] * 1.0e-12
+        E[r] = E[r] * coef
+        E_low[r] = E_low[r] * coef
+    return dict(
+        species_names=species,
+        reactions_type=np.array(rtypes, dtype=np.int32),
+        chemical_coeffs=cc,
+        A=A,
+        beta=np.array(beta, dtype=np.float64),
+        E_act=E,
+        A_low=A_low,
+        beta_low=np.array(beta_low, dtype=np.float64),
+        E_act_low=E_low,
+        Troe_coeffs=np.ascontiguousarray(np.array(troe, dtype=np.float64).T),
+        enhanced_efficiencies=np.ascontiguousarray(np.array(effs, dtype=np.float64).T),
+        reaction_strings=rstrings,
+    )
+
+
+def _nasa7_tokens(line):
+    """Tokenizer of get_a_coeffs (thermophysical_properties_class.f90:1259-1304).
+
+    Yields (token_text, is_last_char_boundary) at each boundary the reference
+    detects: a number ends at a non-number character, a '-' not following 'E'
+    starts a new number, and the last character of the trimmed record ends one.
+    """
+    s = line.rstrip()
+    n = len(s)
+    prev = -1
+    beginning = 0
+    classes = "-+E0123456789."
+    for i in range(n):
+        indx = classes.find(s[i]) + 1
+        boundary = ((indx == 0 and prev > 0)
+                    or (indx == 1 and prev != 3 and prev > 0)
+                    or (indx != 0 and i == n - 1))
+        if boundary:
+            if i != n - 1:
+                yield s[beginning:i]
+            else:
+                yield s[beginning:i + 1]
+            beginning = i
+        prev = indx
+
+
+def _a_coeffs_for(name, lines):
+    """NASA-7 coefficients (7,2) for one species (get_a_coeffs, :1226-1311)."""
+    a = np.zeros((2, 7), dtype=np.float64)     # [set, i], set 0 = low T, 1 = high T
+    k = 0
+    while k < len(lines):
+        items = _list_directed_items(lines[k])
+        k += 1
+        if not items:
+            continue
+        tok = items[0]
+        if tok == "END":
+            break
+        if tok != name:
+            continue
+        num = 0
+        i1, i2 = 1, 2
+        while num < 16:
+            if k >= len(lines):
+                raise MechanismError("unsuccessful specific heat capacity reading " + name)
+            line = lines[k]
+            k += 1
+            for text in _nasa7_tokens(line):
+                if num in (5, 11):
+                    num += 1
+                    continue
+                if num == 16:
+                    break
+                num += 1
+                a[i2 - 1, i1 - 1] = _fortran_real(text)
+                i1 += 1
+                if num == 8:
+                    i1, i2 = 1, 1
+        # the reference keeps scanning after a hit; a later duplicate would overwrite
+    return a
+
+
+def parse_thermo(species, thermo_path, molar_mass_path):
+    """a_coeffs(7,2,S) x R_u and molar_masses(S) (thermophysical_properties_class.f90:137-179)."""
+    tl = _read_lines(thermo_path)
+    S = len(species)
+    a = np.zeros((S, 2, 7), dtype=np.float64)
+    for s, name in enumerate(species):
+        a[s] = _a_coeffs_for(name, tl) * R_GAS_J
+    mm = np.zeros(S, dtype=np.float64)
+    ml = _read_lines(molar_mass_path)
+    for s, name in enumerate(species):
+        found = False
+        for line in ml:
+            items = _list_directed_items(line)
+            if len(items) < 2:
+                raise MechanismError("molar mass is not defined for specie: " + name)
+            if items[0] == name:
+                mm[s] = _fortran_real(items[1])
+                found = True
+                break
+        if not found:
+            raise MechanismError("molar mass is not defined for specie: " + name)
+    return a, mm
+
+
+def load_mechanism(mech_file="KEROMNES.txt", thermo_file="KEROMNES_THERMO.txt",
+                   molar_mass_file="molar_masses.dat", data_dir=DATA_DIR,
+                   default_enhanced_efficiencies=1.0, E_act_units="cal.mol"):
+    """Mechanism + thermo as the reference's problem setup builds them.
+
+    Defaults follow ``package_interface/src/tests/classic_tests/1D_laminar_velocity.f90:327-330``
+    (KEROMNES, ``E_act_units='cal.mol'``, default efficiency 1.0).
+    """
+    d = parse_mechanism(os.path.join(data_dir, mech_file), default_enhanced_efficiencies, E_act_units)
+    a, mm = parse_thermo(d["species_names"], os.path.join(data_dir, thermo_file),
+                         os.path.join(data_dir, molar_mass_file))
+    return Mechanism(a_coeffs=a, molar_masses=mm, **d)
+
+
+# Element composition of the KEROMNES species (for conservation checks and input synthesis).
+ELEMENTS = ("H", "O", "N")
+_COMPOSITION = {
+    "H2": (2, 0, 0), "O2": (0, 2, 0), "N2": (0, 0, 2), "O": (0, 1, 0), "H": (1, 0, 0),
+    "OH": (1, 1, 0), "H2O": (2, 1, 0), "HO2": (1, 2, 0), "H2O2": (2, 2, 0),
+}
+
+
+def element_matrix(species):
+    """(n_elements, S) atom counts."""
+    return np.array([_COMPOSITION[s] for s in species], dtype=np.float64).T
