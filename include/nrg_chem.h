@@ -175,6 +175,18 @@ int nrg_chem_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt,
                       double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
                       int32_t *cell_status, nrg_chem_stats *stats);
 
+/*
+ * nrg_chem_step_dev plus per-cell work counters, the per-call IWORK statistics
+ * of DDRIV3 (slatec/ddriv3.f90 IWORK(3..5): NSTEP, NFE, NJE), one int32 per
+ * inner cell in inner-loop order (i fastest).  All three or none may be NULL.
+ */
+int nrg_chem_step_dev_diag(nrg_chem_t *h, const nrg_grid *grid, double dt,
+                           const double *T, const double *p, const double *rho,
+                           const double *const *Y, const int8_t *bc,
+                           double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                           int32_t *cell_status, nrg_chem_stats *stats,
+                           int32_t *nstep, int32_t *nfe, int32_t *nje);
+
 /* Same contract with host pointers (fields are staged through device buffers). */
 int nrg_chem_step(nrg_chem_t *h, const nrg_grid *grid, double dt,
                   const double *T, const double *p, const double *rho,

@@ -1,0 +1,861 @@
+// chem_device.h -- device code of the gfx950 chemistry step.
+//
+// One cell per lane.  The cell's whole integrator state (Nordsieck history,
+// iteration matrix, rate constants) lives in VGPRs/AGPRs: every array index
+// below is a compile-time constant after unrolling, and the few order- or
+// pivot-dependent accesses are written as select chains.  Mechanism numeric
+// coefficients are read from an LDS copy (MechCoef) shared by the block.
+//
+// Arithmetic follows the reference's evaluation order operation by operation
+// (this translation unit is compiled with -ffp-contract=off), so results agree
+// with NRG's Fortran step to rounding of the transcendental functions.
+// Reference: computing_module/src/current_build/chemical_kinetics_solver.f90
+// (rates :360-493, RHS :495-540, cell step :228-346) and SLATEC SDRIVE
+// (computing_module/src/third_party/slatec/ddriv3.f90, ddstp.f90, ddntl.f90,
+// ddpst.f90, ddcor.f90, ddpsc.f90, ddscl.f90, ddntp.f90, dgefa.f90, dgesl.f90,
+// dnrm2.f90).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "topology_keromnes.h"
+
+#define NRG_DI __device__ __forceinline__
+
+namespace nrg {
+
+constexpr int S = topo::S;
+constexpr int R = topo::R;
+constexpr int MAXORD = 5;
+
+constexpr double R_GAS_J = 8.31446261815324;   // package_library/src/global_data.f90:36
+constexpr double T_REF = 298.15;               // :38
+constexpr double T_CEIL = 6000.0;              // :44
+
+// Mechanism coefficients, staged once per block in LDS.  Quotients by
+// constants that the reference evaluates per call (a_i/i, (-E)/R_u, P_atm/R_u)
+// are precomputed on the host with the same single IEEE division.
+struct MechCoef {
+  double A[R], beta[R], A_low[R], beta_low[R];
+  double mE_R[R], mElow_R[R];    // (-E_act)/R_u, (-E_act_low)/R_u
+  double troe[4][R];             // alpha, T***, T*, T**
+  double one_m_alpha[R];         // 1 - alpha
+  double eff[R][S];              // enhanced efficiencies
+  double a[S][2][7];             // NASA-7 x R_u; [s][set][i], set 1 = T >= 1000 K
+  double hdiv[S][2][5];          // a5/5, a4/4, a3/3, a2/2, a1/1
+  double sdiv[S][2][3];          // a5/4, a4/3, a3/2
+  double W[S];                   // molar masses
+  double h_ref[S];               // h_s(T_ref), T_ref = 298.15 K
+  double el[MAXORD][MAXORD + 1]; // el[nq-1][j-1] = EL(j, nq)   (slatec/ddcst.f90, MINT = 2)
+  double tq[MAXORD][3];          // tq[nq-1][k-1] = TQ(k, nq)
+  double patm_r;                 // P_atm / R_u
+};
+
+// Integrator constants of one call (reference literals + D1MACH-derived values).
+struct BdfConst {
+  double eps, ewt, hmax, uround, sqrtn, sqrt_uround;
+  double br, bl, bp, facmin;     // u**.875, u**.75, u**-.15, u**.78 (slatec/ddpst.f90:54-57)
+  double t_gate;
+  int mxstep, maxord;
+};
+
+NRG_DI double dmax(double a, double b) { return (a >= b) ? a : b; }
+NRG_DI double dmin(double a, double b) { return (a <= b) ? a : b; }
+
+// ---------------------------------------------------------------------------
+// NASA-7 thermochemistry (thermophysical_properties_class.f90:225-300)
+// ---------------------------------------------------------------------------
+NRG_DI double clip_t(double T) { return dmin(T, T_CEIL); }
+
+NRG_DI double h_molar(const MechCoef &mc, double T, int s)
+{
+  const double temp = clip_t(T);
+  const int set = (temp >= 1000.0) ? 1 : 0;
+  const double *d = mc.hdiv[s][set];
+  double hs = d[0];
+  hs = hs * temp + d[1];
+  hs = hs * temp + d[2];
+  hs = hs * temp + d[3];
+  hs = hs * temp + d[4];
+  hs = hs + mc.a[s][set][5] / temp;
+  return hs * temp;
+}
+
+NRG_DI double s_molar(const MechCoef &mc, double T, int s)
+{
+  const double temp = clip_t(T);
+  const int set = (temp >= 1000.0) ? 1 : 0;
+  const double *d = mc.sdiv[s][set];
+  const double *a = mc.a[s][set];
+  double sm = d[0];
+  sm = sm * temp + d[1];
+  sm = sm * temp + d[2];
+  sm = sm * temp + a[1];
+  return (sm * temp + a[6]) + a[0] * log(temp);
+}
+
+NRG_DI double cp_molar(const MechCoef &mc, double T, int s)
+{
+  const double temp = clip_t(T);
+  const double *a = mc.a[s][(temp >= 1000.0) ? 1 : 0];
+  double cp = a[4];
+  cp = cp * temp + a[3];
+  cp = cp * temp + a[2];
+  cp = cp * temp + a[1];
+  cp = cp * temp + a[0];
+  return cp;
+}
+
+NRG_DI double e_molar(const MechCoef &mc, double T, int s)
+{
+  return h_molar(mc, T, s) - R_GAS_J * T;   // unclipped T (:298)
+}
+
+// ---------------------------------------------------------------------------
+// Kinetics: rate constants (chemical_kinetics_solver.f90:360-493)
+// ---------------------------------------------------------------------------
+struct Rates {
+  double kf[R], kr[R], Mc[R];
+};
+
+// [M]_r for the reactions that use it (:235-248): third-body reactions read it
+// in the RHS, fall-off reactions in the reduced pressure.
+NRG_DI constexpr bool uses_m(int r)
+{
+  return topo::rtype[r] == 1 || topo::rtype[r] == 2 || topo::rtype[r] == 3 ||
+         topo::rtype[r] == 6 || topo::rtype[r] == 7;
+}
+
+NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], Rates &k)
+{
+  const double T = dmin(Tin, 10000.0);
+  double sv[S], hv[S];
+#pragma unroll
+  for (int c = 0; c < S; ++c) {
+    sv[c] = s_molar(mc, T, c);
+    hv[c] = h_molar(mc, T, c);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (uses_m(r)) {
+      double m = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) m = m + C[s] * mc.eff[r][s];
+      k.Mc[r] = m;
+    } else {
+      k.Mc[r] = 0.0;
+    }
+    double d_s = 0.0, d_h = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int sp = topo::right[r][c];
+      if (c < topo::nr[r] && sp != S) {
+        d_s = d_s + sv[sp < S ? sp : 0];
+        d_h = d_h + hv[sp < S ? sp : 0];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int sp = topo::left[r][c];
+      if (c < topo::nl[r] && sp != S) {
+        d_s = d_s - sv[sp < S ? sp : 0];
+        d_h = d_h - hv[sp < S ? sp : 0];
+      }
+    }
+    const double d_nu = (double)(topo::nr[r] - topo::nl[r]);
+    const int rt = topo::rtype[r];
+    double kf = 0.0, kr = 0.0;
+    if (rt == 0 || rt == 1 || rt == 5 || rt == 6) {
+      kf = (mc.A[r] * pow(T, mc.beta[r])) * exp(mc.mE_R[r] / T);
+    } else if (rt == 2 || rt == 7) {
+      const double k0 = (mc.A_low[r] * pow(T, mc.beta_low[r])) * exp(mc.mElow_R[r] / T);
+      const double kinf = (mc.A[r] * pow(T, mc.beta[r])) * exp(mc.mE_R[r] / T);
+      const double pr = (k0 * k.Mc[r]) / kinf;
+      kf = kinf * (pr / (1.0 + pr));
+    } else if (rt == 3) {
+      double k0 = exp(mc.mElow_R[r] / T);
+      double kinf = exp(mc.mE_R[r] / T);
+      k0 = (mc.A_low[r] * pow(T, mc.beta_low[r])) * k0;
+      kinf = (mc.A[r] * pow(T, mc.beta[r])) * kinf;
+      const double pr = (k0 * k.Mc[r]) / kinf;
+      const double fcent = (mc.one_m_alpha[r] * exp(-(T / mc.troe[1][r])) +
+                            mc.troe[0][r] * exp(-(T / mc.troe[2][r]))) +
+                           exp(-(mc.troe[3][r] / T));
+      const double lf = log10(fcent);
+      const double c = -0.4 - 0.67 * lf;
+      const double n = 0.75 - 1.27 * lf;
+      const double d = 0.14;
+      double blend;
+      if (pr != 0.0) {
+        const double lp = log10(pr);
+        const double x = (lp + c) / (n - d * (lp + c));
+        blend = pow(fcent, 1.0 / (1.0 + x * x));
+      } else {
+        blend = pow(fcent, -1.0 / d);
+      }
+      kf = (kinf * (pr / (1.0 + pr))) * blend;
+    }
+    if (rt == 0 || rt == 1 || rt == 2 || rt == 3) {
+      const double Kp = exp(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
+      // pow(x, d_nu) with the integral d_nu of this reaction; d_nu = 0 gives 1 exactly.
+      const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * pow(mc.patm_r / T, d_nu);
+      kr = kf / Kc;
+    }
+    k.kf[r] = kf;
+    k.kr[r] = kr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mass-action right-hand side, exactly the accumulation order of fun (:507-536)
+// ---------------------------------------------------------------------------
+NRG_DI void rhs(const Rates &k, const double (&Z)[S], double (&Zd)[S])
+{
+#pragma unroll
+  for (int s = 0; s < S; ++s) Zd[s] = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double fwd = k.kf[r];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int sp = topo::left[r][c];
+      if (c < topo::nl[r]) fwd = fwd * ((sp == S) ? k.Mc[r] : Z[sp < S ? sp : 0]);
+    }
+    double rev = k.kr[r];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int sp = topo::right[r][c];
+      if (c < topo::nr[r]) rev = rev * ((sp == S) ? k.Mc[r] : Z[sp < S ? sp : 0]);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (c < topo::nl[r] && topo::left[r][c] == s) Zd[s] = (Zd[s] - fwd) + rev;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (c < topo::nr[r] && topo::right[r][c] == s) Zd[s] = (Zd[s] - rev) + fwd;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DNRM2 (slatec/dnrm2.f90:84-160): Lawson's phases as a per-element state machine.
+// st: 0 = start (label 30), 1 = phase 1 (50), 2 = phase 2 (70), 3 = phase 4 (110),
+//     4 = phase 3 (the DO 95 loop, unscaled sum).
+// ---------------------------------------------------------------------------
+NRG_DI double dnrm2(const double (&x)[S])
+{
+  const double cutlo = 8.232e-11;
+  const double hitest = 1.304e19 / (double)S;
+  double sum = 0.0, xmax = 0.0;
+  int st = 0;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const double v = x[i], av = fabs(v);
+    bool done = false;
+    if (st == 0) {
+      if (av > cutlo) {
+        st = 4;
+      } else {
+        st = 1;
+        xmax = 0.0;
+      }
+    }
+    if (st == 1) {
+      if (v == 0.0) {
+        done = true;
+      } else if (av > cutlo) {
+        st = 4;
+      } else {
+        st = 2;
+        xmax = av;
+        const double r = v / xmax;
+        sum = sum + r * r;
+        done = true;
+      }
+    }
+    if (!done && st == 2) {
+      if (av > cutlo) {
+        sum = (sum * xmax) * xmax;
+        st = 4;
+      } else {
+        if (av <= xmax) {
+          const double r = v / xmax;
+          sum = sum + r * r;
+        } else {
+          const double r = xmax / v;
+          sum = 1.0 + sum * (r * r);
+          xmax = av;
+        }
+        done = true;
+      }
+    }
+    if (!done && st == 3) {
+      if (av <= xmax) {
+        const double r = v / xmax;
+        sum = sum + r * r;
+      } else {
+        const double r = xmax / v;
+        sum = 1.0 + sum * (r * r);
+        xmax = av;
+      }
+      done = true;
+    }
+    if (!done && st == 4) {
+      if (av >= hitest) {
+        sum = (sum / v) / v;
+        xmax = av;
+        const double r = v / xmax;
+        sum = sum + r * r;
+        st = 3;
+      } else {
+        sum = sum + v * v;
+      }
+    }
+  }
+  return (st == 4) ? sqrt(sum) : xmax * sqrt(sum);
+}
+
+// ---------------------------------------------------------------------------
+// 9x9 LU with partial pivoting (DGEFA) and solve (DGESL), column-major a[col][row].
+// Row interchanges use select chains; a wave skips them when no lane pivots.
+// ---------------------------------------------------------------------------
+NRG_DI bool wave_any(bool p) { return __any(p ? 1 : 0) != 0; }
+
+NRG_DI int gefa(double (&a)[S][S], int (&ipvt)[S])
+{
+  int info = 0;
+#pragma unroll
+  for (int k = 0; k < S - 1; ++k) {
+    int l = k;
+    double dmx = fabs(a[k][k]), piv = a[k][k];
+#pragma unroll
+    for (int i = k + 1; i < S; ++i) {
+      const double xm = fabs(a[k][i]);
+      if (xm > dmx) {
+        l = i;
+        dmx = xm;
+        piv = a[k][i];
+      }
+    }
+    ipvt[k] = l;
+    const bool zero = (piv == 0.0);
+    if (zero) info = k + 1;
+    const bool anyswap = wave_any(l != k);
+    if (anyswap) {
+#pragma unroll
+      for (int i = k + 1; i < S; ++i) a[k][i] = (l == i) ? a[k][k] : a[k][i];
+      a[k][k] = piv;
+    }
+    const double t = -1.0 / a[k][k];
+#pragma unroll
+    for (int i = k + 1; i < S; ++i) a[k][i] = zero ? a[k][i] : t * a[k][i];
+#pragma unroll
+    for (int j = k + 1; j < S; ++j) {
+      double tj = a[j][k];
+      if (anyswap) {
+#pragma unroll
+        for (int i = k + 1; i < S; ++i) tj = (l == i) ? a[j][i] : tj;
+#pragma unroll
+        for (int i = k + 1; i < S; ++i) a[j][i] = (l == i) ? a[j][k] : a[j][i];
+        a[j][k] = tj;
+      }
+      const bool skip = zero || (tj == 0.0);
+#pragma unroll
+      for (int i = k + 1; i < S; ++i) a[j][i] = skip ? a[j][i] : a[j][i] + tj * a[k][i];
+    }
+  }
+  ipvt[S - 1] = S - 1;
+  if (a[S - 1][S - 1] == 0.0) info = S;
+  return info;
+}
+
+NRG_DI void gesl(const double (&a)[S][S], const int (&ipvt)[S], double (&b)[S])
+{
+#pragma unroll
+  for (int k = 0; k < S - 1; ++k) {
+    const int l = ipvt[k];
+    double t = b[k];
+    if (wave_any(l != k)) {
+#pragma unroll
+      for (int i = k + 1; i < S; ++i) t = (l == i) ? b[i] : t;
+#pragma unroll
+      for (int i = k + 1; i < S; ++i) b[i] = (l == i) ? b[k] : b[i];
+      b[k] = t;
+    }
+    const bool skip = (t == 0.0);
+#pragma unroll
+    for (int i = k + 1; i < S; ++i) b[i] = skip ? b[i] : b[i] + t * a[k][i];
+  }
+#pragma unroll
+  for (int k = S - 1; k >= 0; --k) {
+    b[k] = b[k] / a[k][k];
+    const double t = -b[k];
+    const bool skip = (t == 0.0);
+#pragma unroll
+    for (int i = 0; i < k; ++i) b[i] = skip ? b[i] : b[i] + t * a[k][i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gear BDF state of one cell (WORK/IWORK of DDRIV3, private to the lane).
+// ---------------------------------------------------------------------------
+struct Cell {
+  double yh[MAXORD + 1][S];   // yh[j][i] = YH(i, j+1)
+  double y[S], ywt[S], save1[S], save2[S], fac[S];
+  double P[S][S];             // iteration matrix / LU, column-major
+  int ipvt[S];
+  double t, h, rc, rmax, trend;
+  int nq, nwait, jtask, jstepl, nstep, nfe, nje, ncorr, natt;
+  bool convrg, ier;
+};
+
+NRG_DI double el_of(const MechCoef &mc, int j, int nq) { return mc.el[nq - 1][j - 1]; }
+NRG_DI double tq_of(const MechCoef &mc, int k, int nq) { return mc.tq[nq - 1][k - 1]; }
+
+// DDPSC (slatec/ddpsc.f90:26-38) for a compile-time order.
+template <int NQ, bool ADD>
+NRG_DI void psc_n(double (&yh)[MAXORD + 1][S])
+{
+#pragma unroll
+  for (int j1 = 1; j1 <= NQ; ++j1)
+#pragma unroll
+    for (int j2 = j1; j2 <= NQ; ++j2) {
+      const int j = NQ - j2 + j1;
+#pragma unroll
+      for (int i = 0; i < S; ++i) yh[j - 1][i] = ADD ? yh[j - 1][i] + yh[j][i] : yh[j - 1][i] - yh[j][i];
+    }
+}
+
+template <bool ADD>
+NRG_DI void psc(int nq, double (&yh)[MAXORD + 1][S])
+{
+  switch (nq) {
+    case 1: psc_n<1, ADD>(yh); break;
+    case 2: psc_n<2, ADD>(yh); break;
+    case 3: psc_n<3, ADD>(yh); break;
+    case 4: psc_n<4, ADD>(yh); break;
+    default: psc_n<5, ADD>(yh); break;
+  }
+}
+
+// YH(:, NQ+1) read through a select chain.
+NRG_DI void yh_col(const Cell &c, int col, double (&out)[S])
+{
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    double v = c.yh[1][i];
+#pragma unroll
+    for (int j = 2; j <= MAXORD; ++j) v = (col == j) ? c.yh[j][i] : v;
+    out[i] = v;
+  }
+}
+
+// DDSCL (slatec/ddscl.f90:24-35).
+NRG_DI void scl(Cell &c, double rh, const BdfConst &k)
+{
+  const double ah = fabs(c.h);
+  if (c.h < 1.0)
+    rh = dmin(dmin(ah * rh, ah * c.rmax), k.hmax) / ah;
+  else
+    rh = dmin(dmin(rh, c.rmax), k.hmax / ah);
+  double r1 = 1.0;
+#pragma unroll
+  for (int j = 1; j <= MAXORD; ++j) {
+    r1 = r1 * rh;
+    const bool on = (j <= c.nq);
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.yh[j][i] = on ? c.yh[j][i] * r1 : c.yh[j][i];
+  }
+  c.h = c.h * rh;
+  c.rc = c.rc * rh;
+}
+
+template <class F>
+NRG_DI void f_eval(Cell &c, const F &f, const double (&y)[S], double (&out)[S])
+{
+  f(y, out);
+}
+
+// DDNTL, JTASK >= 0 (slatec/ddntl.f90:52-63,152-161).  EL/TQ are precomputed.
+template <class F>
+NRG_DI void ntl(Cell &c, const F &f, const BdfConst &k)
+{
+  c.ier = false;
+  if (c.jtask == 0) c.rmax = 10000.0;
+  c.rc = 0.0;
+  c.convrg = false;
+  c.trend = 1.0;
+  c.nq = 1;
+  c.nwait = 3;
+  f(c.y, c.save2);
+  c.nfe++;
+#pragma unroll
+  for (int i = 0; i < S; ++i) c.save1[i] = c.save2[i] / dmax(1.0, c.ywt[i]);
+  const double sum = dnrm2(c.save1) / k.sqrtn;
+  if (sum > k.eps / fabs(c.h)) c.h = copysign(k.eps / sum, c.h);
+#pragma unroll
+  for (int i = 0; i < S; ++i) c.yh[1][i] = c.h * c.save2[i];
+#pragma unroll
+  for (int i = 0; i < S; ++i) c.fac[i] = k.sqrt_uround;
+}
+
+// DDPST, MITER = 2 (slatec/ddpst.f90:39-114,142-143): finite-difference
+// Jacobian with adaptive increments FAC, P = I - h*l0*J, LU factorisation.
+template <class F>
+NRG_DI void pst(Cell &c, const F &f, const BdfConst &k, const MechCoef &mc)
+{
+  c.nje++;
+  c.ier = false;
+  const double l0 = el_of(mc, 1, c.nq);
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const double ys = dmax(fabs(c.ywt[j]), fabs(c.y[j]));
+    double fj = c.fac[j];
+    double dy = fj * ys;
+    while (dy == 0.0) {
+      if (fj < 0.5) {
+        fj = dmin(100.0 * fj, 0.5);
+        dy = fj * ys;
+      } else {
+        dy = ys;
+        break;
+      }
+    }
+    dy = (c.nq == 1) ? copysign(dy, c.save2[j]) : copysign(dy, c.yh[2][j]);
+    dy = (c.y[j] + dy) - c.y[j];
+    const double yj = c.y[j];
+    c.y[j] = c.y[j] + dy;
+    double f1[S];
+    f(c.y, f1);
+    c.y[j] = yj;
+    const double factor = ((-l0) * c.h) / dy;
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.P[j][i] = (f1[i] - c.save2[i]) * factor;
+    double diff = fabs(c.save2[0] - f1[0]);
+    double s2m = c.save2[0], s1m = f1[0];
+#pragma unroll
+    for (int i = 1; i < S; ++i) {
+      const double di = fabs(c.save2[i] - f1[i]);
+      if (di > diff) {
+        diff = di;
+        s2m = c.save2[i];
+        s1m = f1[i];
+      }
+    }
+    if (dmin(fabs(s2m), fabs(s1m)) > 0.0) {
+      const double scale = dmax(fabs(s2m), fabs(s1m));
+      if (diff > 0.5 * scale)
+        fj = dmax(k.facmin, fj * 0.5);
+      else if (k.br * scale <= diff && diff <= k.bl * scale)
+        fj = dmin(fj * 2.0, 0.5);
+      else if (diff < k.br * scale)
+        fj = dmin(k.bp * fj, 0.5);
+    }
+    c.fac[j] = fj;
+  }
+  c.nfe += S;
+#pragma unroll
+  for (int i = 0; i < S; ++i) c.P[i][i] = c.P[i][i] + 1.0;
+  if (gefa(c.P, c.ipvt) != 0) c.ier = true;
+}
+
+// DDCOR, MITER = 2, IMPL = 0, IERROR = 3 (slatec/ddcor.f90:51-54,98-108).
+NRG_DI double cor(Cell &c, const BdfConst &k)
+{
+#pragma unroll
+  for (int i = 0; i < S; ++i) c.save2[i] = ((c.h * c.save2[i]) - c.yh[1][i]) - c.save1[i];
+  gesl(c.P, c.ipvt, c.save2);
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    c.save1[i] = c.save1[i] + c.save2[i];
+    c.save2[i] = c.save2[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+  }
+  c.ncorr++;
+  return dnrm2(c.save2) / k.sqrtn;
+}
+
+// DDSTP, MINT = 2, no method switching (slatec/ddstp.f90:91-437).  Returns JSTATE.
+template <class F>
+NRG_DI int stp(Cell &c, const F &f, const BdfConst &k, const MechCoef &mc)
+{
+  const double told = c.t;
+  int ntry = 0, nfail = 0;
+  if (c.jtask <= 0) {
+    ntl(c, f, k);
+    if (c.h == 0.0) return 2;
+    if (c.ier) return 4;
+  }
+  for (;;) {                                               // label 100
+    ntry++;
+    if (ntry > 50) return 3;
+    c.natt++;
+    c.t = c.t + c.h;
+    psc<true>(c.nq, c.yh);
+    bool evaljc = (fabs(c.rc - 1.0) > 0.3) || (c.nstep >= c.jstepl + 10);
+    bool converged = false;
+    int iter = 0;
+    double ctest = 0.0;
+    for (;;) {                                             // label 110
+      iter = 0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
+      f(c.y, c.save2);
+      c.nfe++;
+      if (evaljc || c.ier) {
+        pst(c, f, k, mc);
+        if (c.ier) break;                                  // -> 160
+        c.convrg = false;
+        c.rc = 1.0;
+        c.jstepl = c.nstep;
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) c.save1[i] = 0.0;
+      double d1 = 0.0;
+      for (;;) {                                           // label 130
+        const double d = cor(c, k);
+        if (iter > 0) c.trend = dmax(0.9 * c.trend, d / d1);
+        d1 = d;
+        ctest = dmin(2.0 * c.trend, 1.0) * d;
+        if (ctest <= k.eps) {
+          converged = true;
+          break;
+        }
+        iter++;
+        if (iter >= 3) break;
+        const double l0 = el_of(mc, 1, c.nq);
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i] + l0 * c.save1[i];
+        f(c.y, c.save2);
+        c.nfe++;
+      }
+      if (converged) break;
+      if (c.convrg) {                                      // stale Jacobian: refresh, retry
+        evaljc = true;
+        continue;
+      }
+      break;
+    }
+    if (!converged) {                                      // label 160
+      c.t = told;
+      psc<false>(c.nq, c.yh);
+      c.nwait = c.nq + 2;
+      if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
+      const double rh = (iter == 0) ? 0.3 : 0.9 * pow(k.eps / ctest, 0.2);
+      if (rh * c.h == 0.0) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
+        return 2;
+      }
+      scl(c, rh, k);
+      continue;
+    }
+    // label 170: error test
+    c.convrg = true;
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.save2[i] = c.save1[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+    const double etest = dnrm2(c.save2) / (tq_of(mc, 2, c.nq) * k.sqrtn);
+    if (etest > k.eps) {
+      c.t = told;
+      psc<false>(c.nq, c.yh);
+      nfail++;
+      if (nfail < 3 || c.nq == 1) {
+        if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
+        const double rh2 = 1.0 / (1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
+        double rh = rh2;
+        if (c.nq > 1) {
+          double col[S];
+          yh_col(c, c.nq, col);
+#pragma unroll
+          for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+          const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
+          const double rh1 = 1.0 / dmax(1.0, 1.3 * pow(erdn / k.eps, 1.0 / (double)c.nq));
+          if (rh2 < rh1) {
+            c.nq = c.nq - 1;
+            c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
+            rh = rh1;
+          }
+        }
+        c.nwait = c.nq + 2;
+        if (rh * c.h == 0.0) {
+#pragma unroll
+          for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
+          return 2;
+        }
+        scl(c, rh, k);
+        continue;
+      }
+      nfail = 0;
+      c.jtask = 2;
+#pragma unroll
+      for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
+      ntl(c, f, k);
+      c.rmax = 10.0;
+      if (c.h == 0.0) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
+        return 2;
+      }
+      if (c.ier) return 4;
+      continue;
+    }
+    // accepted step (slatec/ddstp.f90:291-300)
+    c.nstep++;
+#pragma unroll
+    for (int j = 1; j <= MAXORD + 1; ++j) {
+      if (j <= c.nq + 1) {
+        const double e = el_of(mc, j, c.nq);
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.yh[j - 1][i] = c.yh[j - 1][i] + e * c.save1[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
+    // step-size and order selection (slatec/ddstp.f90:360-432)
+    if (c.jtask == 0 || c.jtask == 2) {
+      const double rh = 1.0 / dmax(k.uround, 1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
+      if (rh > 1.0) scl(c, rh, k);
+    } else if (c.nwait > 1) {
+      c.nwait = c.nwait - 1;
+      if (c.nwait == 1 && c.nq < k.maxord) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.yh[MAXORD][i] = c.save1[i];
+      }
+    } else {
+      double rh1 = 0.0, rh3 = 0.0;
+      if (c.nq != 1) {
+        double col[S];
+        yh_col(c, c.nq, col);
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+        const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
+        rh1 = 1.0 / dmax(k.uround, 1.3 * pow(erdn / k.eps, 1.0 / (double)c.nq));
+      }
+      const double rh2 = 1.0 / dmax(k.uround, 1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
+      if (c.nq != k.maxord) {
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+          c.save2[i] = (c.save1[i] - c.yh[MAXORD][i]) / dmax(fabs(c.y[i]), c.ywt[i]);
+        const double erup = dnrm2(c.save2) / (tq_of(mc, 3, c.nq) * k.sqrtn);
+        rh3 = 1.0 / dmax(k.uround, 1.4 * pow(erup / k.eps, 1.0 / (double)(c.nq + 2)));
+      }
+      double rh;
+      bool change = true;
+      if (rh1 > rh2 && rh1 >= rh3) {
+        rh = rh1;
+        if (rh <= 1.0) {
+          change = false;
+        } else {
+          c.nq = c.nq - 1;
+          c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
+        }
+      } else if (rh2 >= rh1 && rh2 >= rh3) {
+        rh = rh2;
+        if (rh <= 1.0) change = false;
+      } else {
+        rh = rh3;
+        if (rh <= 1.0) {
+          change = false;
+        } else {
+          const double e = el_of(mc, c.nq + 1, c.nq);
+          const double den = (double)(c.nq + 1);
+          const int dst = c.nq + 1;
+#pragma unroll
+          for (int j = 2; j <= MAXORD; ++j) {
+#pragma unroll
+            for (int i = 0; i < S; ++i) c.yh[j][i] = (dst == j) ? (c.save1[i] * e) / den : c.yh[j][i];
+          }
+          c.nq = c.nq + 1;
+          c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq - 1);
+        }
+      }
+      if (change) {
+        scl(c, rh, k);
+        c.rmax = 10.0;
+      }
+      c.nwait = c.nq + 2;
+    }
+    return 1;
+  }
+}
+
+// DDNTP, K = 0 (slatec/ddntp.f90:25-32).
+NRG_DI void ntp(const Cell &c, double tout, double (&y)[S])
+{
+  double col[S];
+  yh_col(c, c.nq, col);
+#pragma unroll
+  for (int i = 0; i < S; ++i) y[i] = col[i];
+  const double r = (tout - c.t) / c.h;
+#pragma unroll
+  for (int j = MAXORD; j >= 1; --j) {
+    const bool on = (j <= c.nq);
+#pragma unroll
+    for (int i = 0; i < S; ++i) y[i] = on ? c.yh[j - 1][i] + r * y[i] : y[i];
+  }
+}
+
+// Per-cell outcome of the integration.
+enum CellOutcome { OUT_OK = 0, OUT_MXSTEP = 2, OUT_STEP_ZERO = 3, OUT_REPEATED_FAIL = 4,
+                   OUT_SINGULAR = 5, OUT_EPS = 6 };
+
+// DDRIV3 with the chemistry solver's settings (NSTATE=1, NTASK=1, NROOT=0,
+// IERROR=3, MINT=2, MITER=2, IMPL=0) from t = 0 to tout, y in/out.
+// slatec/ddriv3.f90:930-982 (init), :1292-1341 (YWT, guards), :1351-1449 (loop).
+template <class F>
+NRG_DI int ddriv3(Cell &c, const F &f, double (&y)[S], double tout, const BdfConst &k,
+                  const MechCoef &mc)
+{
+  c.t = 0.0;
+  double h = (tout - c.t) * (1.0 - 4.0 * k.uround);
+  h = copysign(dmin(fabs(h), k.hmax), h);
+  c.h = h;
+  const double hsign = copysign(1.0, h);
+  c.jtask = 0;
+  c.nstep = c.nfe = c.nje = c.ncorr = c.natt = 0;
+  c.jstepl = 0;
+  c.ier = false;
+  c.rc = 0.0;
+  c.rmax = 0.0;
+  c.trend = 0.0;
+  c.nq = 1;
+  c.nwait = 0;
+  c.convrg = false;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    c.yh[0][i] = y[i];
+    c.y[i] = y[i];
+  }
+  if (c.t == tout) return OUT_OK;
+  for (;;) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.ywt[i] = dmax(k.ewt, fabs(c.y[i]));
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.save2[i] = c.y[i] / c.ywt[i];
+    double sum = dnrm2(c.save2) / k.sqrtn;
+    sum = dmax(1.0, sum);
+    if (k.eps < sum * k.uround) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) y[i] = c.yh[0][i];
+      return OUT_EPS;
+    }
+    if (c.nstep == k.mxstep) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) y[i] = c.yh[0][i];
+      return OUT_MXSTEP;
+    }
+    const int js = stp(c, f, k, mc);
+    if (js != 1) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) y[i] = c.y[i];
+      return (js == 2) ? OUT_STEP_ZERO : (js == 3) ? OUT_REPEATED_FAIL : OUT_SINGULAR;
+    }
+    c.jtask = 1;
+    if (c.t * hsign < tout * hsign) continue;
+    ntp(c, tout, y);
+    return OUT_OK;
+  }
+}
+
+}  // namespace nrg

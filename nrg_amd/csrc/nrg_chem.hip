@@ -1,0 +1,825 @@
+// nrg_chem.hip -- kernels and C-ABI of the gfx950 chemistry step.
+//
+// Entry points are declared in include/nrg_chem.h; each replaces a piece of
+// NRG's chemical_kinetics_solver (computing_module/src/current_build/
+// chemical_kinetics_solver.f90) or of the split-step closure the flow solvers
+// apply after it.  All device work runs on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/nrg_chem.h"
+#include "chem_device.h"
+
+using namespace nrg;
+
+namespace {
+
+thread_local char g_last_error[512] = "";
+
+int set_error(int status, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+  return status;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_error(NRG_CHEM_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+  } while (0)
+
+// Field addressing of one call (Fortran column-major with ghost layers).
+struct GridIdx {
+  int lo0, lo1, lo2;
+  int nx, ny;                 // allocated extents in dims 1, 2
+  int ilo0, ilo1, ilo2;       // inner lower bounds
+  int ix, iy, iz;             // inner extents
+  long long ninner;
+};
+
+NRG_DI long long cell_offset(const GridIdx &g, long long q)
+{
+  const long long i = q % g.ix + g.ilo0;
+  const long long j = (q / g.ix) % g.iy + g.ilo1;
+  const long long kk = q / ((long long)g.ix * g.iy) + g.ilo2;
+  return (i - g.lo0) + (long long)g.nx * ((j - g.lo1) + (long long)g.ny * (kk - g.lo2));
+}
+
+struct StepArgs {
+  GridIdx g;
+  double dt;
+  BdfConst k;
+  const MechCoef *mech;
+  const double *T, *rho;
+  const double *Y[S];
+  const int8_t *bc;
+  double *Yp[S], *Yp2[S];
+  double *Efp;
+  int32_t *status;
+  int32_t *nstep, *nfe, *nje;          // optional per-cell work counters
+  unsigned long long *acc;              // [16] work/status accumulators
+};
+
+// Block-wide staging of the mechanism coefficients into LDS.
+NRG_DI void stage_mech(MechCoef &dst, const MechCoef *src)
+{
+  const double *s = reinterpret_cast<const double *>(src);
+  double *d = reinterpret_cast<double *>(&dst);
+  constexpr int n = sizeof(MechCoef) / sizeof(double);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = s[i];
+  __syncthreads();
+}
+
+NRG_DI unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+NRG_DI int wave_max_i32(int v)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// accumulator slots
+enum { ACC_INTEG = 0, ACC_NSTEP, ACC_NFE, ACC_NJE, ACC_NCORR, ACC_NATT, ACC_NSTEP_MAX, ACC_NFE_MAX,
+       ACC_STATUS0 };   // ACC_STATUS0 + NRG_CELL_* (8 slots)
+
+// One chemistry step, one inner cell per lane (solve_chemical_kinetics, :224-346).
+__global__ void __launch_bounds__(256) chem_step_kernel(StepArgs a)
+{
+  __shared__ MechCoef mc;
+  stage_mech(mc, a.mech);
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = q < a.g.ninner;
+  int status = NRG_CELL_GATED;
+  int nstep = 0, nfe = 0, nje = 0, ncorr = 0, natt = 0;
+  if (live) {
+    const long long off = cell_offset(a.g, q);
+    const double T = a.T[off];
+    const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
+    if (T >= a.k.t_gate && bcv == 0) {
+      const double rho = a.rho[off];
+      double cin[S], y[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        cin[s] = (a.Y[s][off] * rho) / mc.W[s];
+        y[s] = cin[s];
+      }
+      Rates kr;
+      rate_constants(mc, T, cin, kr);
+      Cell c;
+      auto f = [&kr](const double (&z)[S], double (&zd)[S]) { rhs(kr, z, zd); };
+      const int out = ddriv3(c, f, y, a.dt, a.k, mc);
+      double ef = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const double dc = y[s] - cin[s];
+        a.Yp[s][off] = (dc * mc.W[s]) / a.dt;
+        a.Yp2[s][off] = dc;
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s) ef = ef - (mc.h_ref[s] * (y[s] - cin[s])) / a.dt;
+      a.Efp[off] = ef;
+      status = (out == OUT_OK) ? NRG_CELL_OK
+             : (out == OUT_MXSTEP) ? NRG_CELL_MXSTEP
+             : (out == OUT_STEP_ZERO) ? NRG_CELL_STEP_ZERO
+             : (out == OUT_REPEATED_FAIL) ? NRG_CELL_REPEATED_FAIL
+             : NRG_CELL_SINGULAR;
+      nstep = c.nstep;
+      nfe = c.nfe;
+      nje = c.nje;
+      ncorr = c.ncorr;
+      natt = c.natt;
+    } else {
+#pragma unroll
+      for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
+      a.Efp[off] = 0.0;
+    }
+    if (a.status) a.status[off] = status;
+    if (a.nstep) {
+      a.nstep[q] = nstep;
+      a.nfe[q] = nfe;
+      a.nje[q] = nje;
+    }
+  }
+  // work counters: one atomic per wave and slot
+  const bool integ = live && status != NRG_CELL_GATED;
+  const unsigned long long vals[6] = {integ ? 1ull : 0ull, (unsigned long long)nstep,
+                                      (unsigned long long)nfe, (unsigned long long)nje,
+                                      (unsigned long long)ncorr, (unsigned long long)natt};
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const unsigned long long w = wave_sum_u64(vals[i]);
+    if (lane == 0 && w) atomicAdd(&a.acc[ACC_INTEG + i], w);
+  }
+  const int mx_step = wave_max_i32(nstep), mx_fe = wave_max_i32(nfe);
+  if (lane == 0) {
+    atomicMax(&a.acc[ACC_NSTEP_MAX], (unsigned long long)mx_step);
+    atomicMax(&a.acc[ACC_NFE_MAX], (unsigned long long)mx_fe);
+  }
+#pragma unroll
+  for (int st = 0; st < 6; ++st) {
+    const unsigned long long w = wave_sum_u64((live && status == st) ? 1ull : 0ull);
+    if (lane == 0 && w) atomicAdd(&a.acc[ACC_STATUS0 + st], w);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Split-step closure and EOS (cabaret_solver.f90:3316-3318,3386-3416;
+// table_approximated_real_gas.f90:410-440; thermophysical_properties_class.f90:409-452)
+// ---------------------------------------------------------------------------
+struct ClosureArgs {
+  GridIdx g;
+  double dt;
+  int apply_sources;                    // 0: EOS only
+  int ndim;
+  const MechCoef *mech;
+  const double *rho;
+  const double *v[3];
+  const double *Efp;
+  const double *Yp[S];
+  const int8_t *bc;
+  double *E_f;
+  double *Y[S];
+  double *T, *p;
+};
+
+NRG_DI double calculate_temperature(const MechCoef &mc, double T, double e_i, const double (&X)[S])
+{
+  const double eps = (double)1e-08f;
+  double temp = clip_t(T);
+  double h_const = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) h_const = h_const + mc.h_ref[s] * X[s];
+  double cp = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) cp = cp + cp_molar(mc, temp, s) * X[s];
+  for (int iter = 1; iter <= 1000; ++iter) {
+    double e = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) e = e + e_molar(mc, temp, s) * X[s];
+    const double e_int = e - h_const;
+    const double residual = (e_i - e_int) / cp;
+    temp = temp + residual;
+    if (fabs(residual) < eps) break;
+  }
+  return temp;
+}
+
+__global__ void __launch_bounds__(256) closure_kernel(ClosureArgs a)
+{
+  __shared__ MechCoef mc;
+  stage_mech(mc, a.mech);
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.g.ninner) return;
+  const long long off = cell_offset(a.g, q);
+  if (a.bc && a.bc[off] != 0) return;
+  const double rho = a.rho[off];
+  double Ef = a.E_f[off];
+  double y[S];
+  if (a.apply_sources) {
+    const double efp = 0.0 + a.Efp[off] * a.dt;
+    Ef = Ef + efp / rho;
+    double sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double ypr = 0.0 + a.Yp[s][off] * a.dt;
+      y[s] = a.Y[s][off] + ypr / rho;
+      sum = sum + y[s];
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) y[s] = dmax(y[s], 0.0) / sum;
+    a.E_f[off] = Ef;
+#pragma unroll
+    for (int s = 0; s < S; ++s) a.Y[s][off] = y[s];
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) y[s] = a.Y[s][off];
+  }
+  double e_i = Ef;
+  for (int d = 0; d < a.ndim; ++d) {
+    const double vd = a.v[d][off];
+    e_i = e_i - 0.5 * (vd * vd);
+  }
+  double avg = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) avg = avg + y[s] / mc.W[s];
+  const double wmix = 1.0 / avg;
+  double X[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) X[s] = (y[s] / mc.W[s]) * wmix;
+  e_i = e_i * wmix;
+  const double Tn = calculate_temperature(mc, a.T[off], e_i, X);
+  a.T[off] = Tn;
+  if (a.p) a.p[off] = ((Tn * rho) * R_GAS_J) / wmix;
+}
+
+struct EnergyArgs {
+  GridIdx g;
+  int ndim;
+  const MechCoef *mech;
+  const double *T;
+  const double *Y[S];
+  const double *v[3];
+  double *E_f;
+};
+
+__global__ void __launch_bounds__(256) energy_kernel(EnergyArgs a)
+{
+  __shared__ MechCoef mc;
+  stage_mech(mc, a.mech);
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.g.ninner) return;
+  const long long off = cell_offset(a.g, q);
+  double y[S], X[S], avg = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    y[s] = a.Y[s][off];
+    avg = avg + y[s] / mc.W[s];
+  }
+  const double wmix = 1.0 / avg;
+#pragma unroll
+  for (int s = 0; s < S; ++s) X[s] = (y[s] / mc.W[s]) * wmix;
+  const double T = a.T[off];
+  double e = 0.0, hc = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) e = e + e_molar(mc, T, s) * X[s];
+#pragma unroll
+  for (int s = 0; s < S; ++s) hc = hc + mc.h_ref[s] * X[s];
+  double Ef = (e - hc) / wmix;
+  for (int d = 0; d < a.ndim; ++d) {
+    const double vd = a.v[d][off];
+    Ef = Ef + 0.5 * (vd * vd);
+  }
+  a.E_f[off] = Ef;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct nrg_chem_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  MechCoef host_mech{};
+  MechCoef *dev_mech = nullptr;
+  BdfConst k{};
+  nrg_chem_opts opts{};
+  unsigned long long *dev_acc = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int block = 256;
+  // staging buffers for the host-pointer entry point
+  std::vector<void *> stage;
+  size_t stage_cells = 0;
+};
+
+namespace {
+
+// Gear coefficient tables EL, TQ for MINT = 2 (slatec/ddcst.f90:39-86): the
+// l-vectors of the BDF formulas of orders 1..5, normalised by l1.
+void gear_tables(double el[MAXORD][MAXORD + 1], double tq[MAXORD][3])
+{
+  double E[14][13] = {};      // E[i][j] = EL(i, j), 1-based
+  double fact[13] = {};
+  fact[1] = 1.0;
+  for (int i = 2; i <= MAXORD; ++i) fact[i] = (double)i * fact[i - 1];
+  E[1][1] = 1.0;
+  E[2][1] = 1.0;
+  for (int j = 2; j <= MAXORD; ++j) {
+    E[1][j] = fact[j];
+    for (int i = 2; i <= j; ++i) E[i][j] = (double)j * E[i][j - 1] + E[i - 1][j - 1];
+    E[j + 1][j] = 1.0;
+  }
+  double harm = 1.0;
+  for (int j = 2; j <= MAXORD; ++j) {
+    harm = harm + 1.0 / (double)j;
+    for (int i = 1; i <= j + 1; ++i) E[i][j] = E[i][j] / (fact[j] * harm);
+  }
+  for (int nq = 1; nq <= MAXORD; ++nq) {
+    for (int j = 1; j <= MAXORD + 1; ++j) el[nq - 1][j - 1] = E[j][nq];
+    tq[nq - 1][0] = (nq > 1) ? 1.0 / fact[nq - 1] : 0.0;
+    tq[nq - 1][1] = (double)(nq + 1) / E[1][nq];
+    tq[nq - 1][2] = (double)(nq + 2) / E[1][nq];
+  }
+}
+
+int build_mech(const nrg_chem_mech *m, MechCoef &mc)
+{
+  if (!m) return set_error(NRG_CHEM_E_ARG, "mech is NULL");
+  if (m->species_number != S || m->reactions_number != R)
+    return set_error(NRG_CHEM_E_MECH, "compiled for S=%d R=%d, got S=%d R=%d", S, R,
+                     m->species_number, m->reactions_number);
+  if (!m->reactions_type || !m->chemical_coeffs || !m->A || !m->beta || !m->E_act || !m->A_low ||
+      !m->beta_low || !m->E_act_low || !m->Troe_coeffs || !m->enhanced_efficiencies ||
+      !m->a_coeffs || !m->molar_masses)
+    return set_error(NRG_CHEM_E_ARG, "mechanism array pointer is NULL");
+  for (int r = 0; r < R; ++r)
+    if (m->reactions_type[r] != topo::rtype[r])
+      return set_error(NRG_CHEM_E_MECH, "reaction %d has type %d, kernels compiled for %d", r + 1,
+                       m->reactions_type[r], topo::rtype[r]);
+  for (int i = 0; i < 8 * R; ++i)
+    if (m->chemical_coeffs[i] != topo::chemical_coeffs[i])
+      return set_error(NRG_CHEM_E_MECH, "chemical_coeffs differ from the compiled topology at %d", i);
+  for (int s = 0; s < S; ++s)
+    if (!(m->molar_masses[s] > 0.0))
+      return set_error(NRG_CHEM_E_MECH, "molar mass of species %d is not positive", s + 1);
+  std::memset(&mc, 0, sizeof(mc));
+  for (int r = 0; r < R; ++r) {
+    mc.A[r] = m->A[r];
+    mc.beta[r] = m->beta[r];
+    mc.A_low[r] = m->A_low[r];
+    mc.beta_low[r] = m->beta_low[r];
+    mc.mE_R[r] = (-m->E_act[r]) / R_GAS_J;
+    mc.mElow_R[r] = (-m->E_act_low[r]) / R_GAS_J;
+    for (int q = 0; q < 4; ++q) mc.troe[q][r] = m->Troe_coeffs[r + R * q];
+    mc.one_m_alpha[r] = 1.0 - mc.troe[0][r];
+    for (int s = 0; s < S; ++s) mc.eff[r][s] = m->enhanced_efficiencies[r + R * s];
+  }
+  for (int s = 0; s < S; ++s) {
+    for (int set = 0; set < 2; ++set) {
+      const double *a = m->a_coeffs + 14 * s + 7 * set;
+      for (int i = 0; i < 7; ++i) mc.a[s][set][i] = a[i];
+      mc.hdiv[s][set][0] = a[4] / 5.0;
+      mc.hdiv[s][set][1] = a[3] / 4.0;
+      mc.hdiv[s][set][2] = a[2] / 3.0;
+      mc.hdiv[s][set][3] = a[1] / 2.0;
+      mc.hdiv[s][set][4] = a[0] / 1.0;
+      mc.sdiv[s][set][0] = a[4] / 4.0;
+      mc.sdiv[s][set][1] = a[3] / 3.0;
+      mc.sdiv[s][set][2] = a[2] / 2.0;
+    }
+    mc.W[s] = m->molar_masses[s];
+  }
+  // h_s(T_ref): low-T set (T_ref < 1000 K), same Horner order as the kernel.
+  for (int s = 0; s < S; ++s) {
+    const double t = T_REF;
+    const double *d = mc.hdiv[s][0];
+    double hs = d[0];
+    for (int i = 1; i < 5; ++i) hs = hs * t + d[i];
+    hs = hs + mc.a[s][0][5] / t;
+    mc.h_ref[s] = hs * t;
+  }
+  gear_tables(mc.el, mc.tq);
+  mc.patm_r = 101325.0 / R_GAS_J;
+  return NRG_CHEM_OK;
+}
+
+BdfConst make_const(const nrg_chem_opts &o)
+{
+  BdfConst k{};
+  k.eps = o.eps;
+  k.ewt = o.ewt;
+  k.hmax = o.hmax;
+  k.uround = 2.220446049250313080847e-16;        // epsilon(1d0)
+  k.sqrtn = std::sqrt((double)S);
+  k.sqrt_uround = std::sqrt(k.uround);
+  k.br = std::pow(k.uround, 0.875);
+  k.bl = std::pow(k.uround, 0.75);
+  k.bp = std::pow(k.uround, -0.15);
+  k.facmin = std::pow(k.uround, 0.78);
+  k.t_gate = o.t_gate;
+  k.mxstep = o.mxstep;
+  k.maxord = std::min(o.mxord, MAXORD);
+  return k;
+}
+
+int make_grid(const nrg_grid *grid, GridIdx &g)
+{
+  if (!grid) return set_error(NRG_CHEM_E_ARG, "grid is NULL");
+  for (int d = 0; d < 3; ++d) {
+    if (grid->hi[d] < grid->lo[d]) return set_error(NRG_CHEM_E_ARG, "grid hi < lo in dim %d", d);
+    if (grid->inner_lo[d] < grid->lo[d] || grid->inner_hi[d] > grid->hi[d])
+      return set_error(NRG_CHEM_E_ARG, "inner bounds outside the field bounds in dim %d", d);
+  }
+  g.lo0 = grid->lo[0];
+  g.lo1 = grid->lo[1];
+  g.lo2 = grid->lo[2];
+  g.nx = grid->hi[0] - grid->lo[0] + 1;
+  g.ny = grid->hi[1] - grid->lo[1] + 1;
+  g.ilo0 = grid->inner_lo[0];
+  g.ilo1 = grid->inner_lo[1];
+  g.ilo2 = grid->inner_lo[2];
+  g.ix = std::max(0, grid->inner_hi[0] - grid->inner_lo[0] + 1);
+  g.iy = std::max(0, grid->inner_hi[1] - grid->inner_lo[1] + 1);
+  g.iz = std::max(0, grid->inner_hi[2] - grid->inner_lo[2] + 1);
+  g.ninner = (long long)g.ix * g.iy * g.iz;
+  return NRG_CHEM_OK;
+}
+
+long long field_cells(const nrg_grid *grid)
+{
+  long long n = 1;
+  for (int d = 0; d < 3; ++d) n *= (long long)(grid->hi[d] - grid->lo[d] + 1);
+  return n;
+}
+
+int check_device(int dev)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return set_error(NRG_CHEM_E_NO_DEVICE, "no HIP device");
+  if (dev < 0 || dev >= n) return set_error(NRG_CHEM_E_NO_DEVICE, "device %d out of range", dev);
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, dev));
+  if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+    return set_error(NRG_CHEM_E_NO_DEVICE, "device %d is %s, kernels are built for gfx950", dev,
+                     p.gcnArchName);
+  return NRG_CHEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nrg_chem_abi_version(void) { return NRG_CHEM_ABI_VERSION; }
+
+void nrg_chem_default_opts(nrg_chem_opts *o)
+{
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->eps = 1.0e-9;                    // chemical_kinetics_solver.f90:274
+  o->ewt = 1.0e-4;                    // :275
+  o->hmax = (double)1.0e-7f;          // :287, default-real literal
+  o->t_gate = 305.0;                  // :228
+  o->mxstep = 10000;                  // :288
+  o->mxord = 5;                       // :282
+  o->jacobian = NRG_JAC_FD_SLATEC;
+  o->device = 0;
+  o->block_size = 0;
+}
+
+const char *nrg_chem_strerror(int st)
+{
+  switch (st) {
+    case NRG_CHEM_OK: return "ok";
+    case NRG_CHEM_E_ARG: return "invalid argument";
+    case NRG_CHEM_E_MECH: return "mechanism not supported by the compiled kernels";
+    case NRG_CHEM_E_HIP: return "HIP runtime error";
+    case NRG_CHEM_E_NO_DEVICE: return "no gfx950 device";
+    case NRG_CHEM_E_STEP_ZERO: return "integrator step size went to zero in a cell (SLATEC IERFLG 41)";
+    case NRG_CHEM_E_REPEATED_FAIL: return "integrator failed 50 attempts in a cell (SLATEC IERFLG 42)";
+    default: return "unknown status";
+  }
+}
+
+const char *nrg_chem_last_error(void) { return g_last_error; }
+
+int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_chem_t **out)
+{
+  if (!out) return set_error(NRG_CHEM_E_ARG, "out is NULL");
+  *out = nullptr;
+  nrg_chem_opts o;
+  if (opts)
+    o = *opts;
+  else
+    nrg_chem_default_opts(&o);
+  if (!(o.eps > 0.0) || !(o.ewt > 0.0) || !(o.hmax > 0.0) || o.mxstep <= 0 || o.mxord <= 0)
+    return set_error(NRG_CHEM_E_ARG, "invalid integrator options");
+  if (o.jacobian != NRG_JAC_FD_SLATEC)
+    return set_error(NRG_CHEM_E_ARG, "jacobian mode %d not available", o.jacobian);
+  if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64))
+    return set_error(NRG_CHEM_E_ARG, "block_size must be 64, 128, 192 or 256");
+  int st = check_device(o.device);
+  if (st) return st;
+  auto *h = new (std::nothrow) nrg_chem_handle();
+  if (!h) return set_error(NRG_CHEM_E_ARG, "out of host memory");
+  st = build_mech(mech, h->host_mech);
+  if (st) {
+    delete h;
+    return st;
+  }
+  h->opts = o;
+  h->device = o.device;
+  h->k = make_const(o);
+  h->block = o.block_size ? o.block_size : 256;
+  auto fail = [&](int s) {
+    nrg_chem_destroy(h);
+    return s;
+  };
+  if (hipSetDevice(h->device) != hipSuccess) return fail(set_error(NRG_CHEM_E_HIP, "hipSetDevice"));
+  if (hipStreamCreate(&h->stream) != hipSuccess) return fail(set_error(NRG_CHEM_E_HIP, "hipStreamCreate"));
+  if (hipMalloc(&h->dev_mech, sizeof(MechCoef)) != hipSuccess)
+    return fail(set_error(NRG_CHEM_E_HIP, "hipMalloc mech"));
+  if (hipMalloc(&h->dev_acc, 16 * sizeof(unsigned long long)) != hipSuccess)
+    return fail(set_error(NRG_CHEM_E_HIP, "hipMalloc acc"));
+  if (hipMemcpy(h->dev_mech, &h->host_mech, sizeof(MechCoef), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_error(NRG_CHEM_E_HIP, "hipMemcpy mech"));
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return fail(set_error(NRG_CHEM_E_HIP, "hipEventCreate"));
+  *out = h;
+  return NRG_CHEM_OK;
+}
+
+void nrg_chem_destroy(nrg_chem_t *h)
+{
+  if (!h) return;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (void *p : h->stage)
+    if (p) hipFree(p);
+  if (h->dev_mech) hipFree(h->dev_mech);
+  if (h->dev_acc) hipFree(h->dev_acc);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+void *nrg_chem_stream(nrg_chem_t *h) { return h ? (void *)h->stream : nullptr; }
+
+static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                         const double *rho, const double *const *Y, const int8_t *bc,
+                         double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                         int32_t *cell_status, nrg_chem_stats *stats, int32_t *nstep,
+                         int32_t *nfe, int32_t *nje)
+{
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return set_error(NRG_CHEM_E_ARG, "time_step must be > 0");
+  if (!T || !rho || !Y || !Y_prod || !Y_prod2 || !E_f_prod)
+    return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL");
+  StepArgs a{};
+  int st = make_grid(grid, a.g);
+  if (st) return st;
+  for (int s = 0; s < S; ++s) {
+    if (!Y[s] || !Y_prod[s] || !Y_prod2[s]) return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", s + 1);
+    a.Y[s] = Y[s];
+    a.Yp[s] = Y_prod[s];
+    a.Yp2[s] = Y_prod2[s];
+  }
+  a.dt = dt;
+  a.k = h->k;
+  a.mech = h->dev_mech;
+  a.T = T;
+  a.rho = rho;
+  a.bc = bc;
+  a.Efp = E_f_prod;
+  a.status = cell_status;
+  a.nstep = nstep;
+  a.nfe = nfe;
+  a.nje = nje;
+  a.acc = h->dev_acc;
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemsetAsync(h->dev_acc, 0, 16 * sizeof(unsigned long long), h->stream));
+  unsigned long long acc[16] = {};
+  float ms = 0.f;
+  if (a.g.ninner > 0) {
+    const long long nb = (a.g.ninner + h->block - 1) / h->block;
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    hipLaunchKernelGGL(chem_step_kernel, dim3((unsigned)nb), dim3(h->block), 0, h->stream, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipMemcpyAsync(acc, h->dev_acc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->cells_inner = a.g.ninner;
+    stats->cells_integrated = (int64_t)acc[ACC_INTEG];
+    stats->nstep_sum = (int64_t)acc[ACC_NSTEP];
+    stats->nfe_sum = (int64_t)acc[ACC_NFE];
+    stats->nje_sum = (int64_t)acc[ACC_NJE];
+    stats->ncorr_sum = (int64_t)acc[ACC_NCORR];
+    stats->natt_sum = (int64_t)acc[ACC_NATT];
+    stats->nstep_max = (int32_t)acc[ACC_NSTEP_MAX];
+    stats->nfe_max = (int32_t)acc[ACC_NFE_MAX];
+    for (int i = 0; i < 6; ++i) stats->status_count[i] = (int64_t)acc[ACC_STATUS0 + i];
+    stats->kernel_ms = ms;
+  }
+  if (acc[ACC_STATUS0 + NRG_CELL_STEP_ZERO])
+    return set_error(NRG_CHEM_E_STEP_ZERO, "%llu cell(s): step size went to zero",
+                     acc[ACC_STATUS0 + NRG_CELL_STEP_ZERO]);
+  if (acc[ACC_STATUS0 + NRG_CELL_REPEATED_FAIL] || acc[ACC_STATUS0 + NRG_CELL_SINGULAR])
+    return set_error(NRG_CHEM_E_REPEATED_FAIL, "%llu cell(s): repeated integrator failures",
+                     acc[ACC_STATUS0 + NRG_CELL_REPEATED_FAIL] + acc[ACC_STATUS0 + NRG_CELL_SINGULAR]);
+  return NRG_CHEM_OK;
+}
+
+int nrg_chem_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                      const double *p, const double *rho, const double *const *Y,
+                      const int8_t *bc, double *const *Y_prod, double *const *Y_prod2,
+                      double *E_f_prod, int32_t *cell_status, nrg_chem_stats *stats)
+{
+  (void)p;   // read but unused by the reference step (chemical_kinetics_solver.f90:307)
+  return step_dev_impl(h, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status, stats,
+                       nullptr, nullptr, nullptr);
+}
+
+int nrg_chem_step_dev_diag(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                           const double *p, const double *rho, const double *const *Y,
+                           const int8_t *bc, double *const *Y_prod, double *const *Y_prod2,
+                           double *E_f_prod, int32_t *cell_status, nrg_chem_stats *stats,
+                           int32_t *nstep, int32_t *nfe, int32_t *nje)
+{
+  (void)p;
+  if ((nstep || nfe || nje) && !(nstep && nfe && nje))
+    return set_error(NRG_CHEM_E_ARG, "nstep, nfe and nje must be all NULL or all set");
+  return step_dev_impl(h, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status, stats,
+                       nstep, nfe, nje);
+}
+
+int nrg_chem_step(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                  const double *p, const double *rho, const double *const *Y, const int8_t *bc,
+                  double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                  int32_t *cell_status, nrg_chem_stats *stats)
+{
+  (void)p;
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (!grid || !T || !rho || !Y || !Y_prod || !Y_prod2 || !E_f_prod)
+    return set_error(NRG_CHEM_E_ARG, "required pointer is NULL");
+  const long long n = field_cells(grid);
+  if (n <= 0) return set_error(NRG_CHEM_E_ARG, "empty field");
+  HIP_TRY(hipSetDevice(h->device));
+  // staging layout: T, rho, E_f_prod, Y[S], Y_prod[S], Y_prod2[S] (doubles), bc (int8), status (int32)
+  const int nd = 3 + 3 * S;
+  if (h->stage_cells < (size_t)n) {
+    for (void *q : h->stage)
+      if (q) hipFree(q);
+    h->stage.assign(nd + 2, nullptr);
+    for (int i = 0; i < nd; ++i) HIP_TRY(hipMalloc(&h->stage[i], n * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->stage[nd], n * sizeof(int8_t)));
+    HIP_TRY(hipMalloc(&h->stage[nd + 1], n * sizeof(int32_t)));
+    h->stage_cells = n;
+  }
+  auto D = [&](int i) { return (double *)h->stage[i]; };
+  const size_t bytes = n * sizeof(double);
+  hipStream_t s = h->stream;
+  HIP_TRY(hipMemcpyAsync(D(0), T, bytes, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(D(1), rho, bytes, hipMemcpyHostToDevice, s));
+  const double *Yd[S];
+  double *Ypd[S], *Yp2d[S];
+  for (int q = 0; q < S; ++q) {
+    if (!Y[q] || !Y_prod[q] || !Y_prod2[q]) return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", q + 1);
+    HIP_TRY(hipMemcpyAsync(D(3 + q), Y[q], bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(D(3 + 2 * S + q), Y_prod2[q], bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(D(3 + S + q), Y_prod[q], bytes, hipMemcpyHostToDevice, s));
+    Yd[q] = D(3 + q);
+    Ypd[q] = D(3 + S + q);
+    Yp2d[q] = D(3 + 2 * S + q);
+  }
+  HIP_TRY(hipMemcpyAsync(D(2), E_f_prod, bytes, hipMemcpyHostToDevice, s));
+  int8_t *bcd = nullptr;
+  if (bc) {
+    bcd = (int8_t *)h->stage[nd];
+    HIP_TRY(hipMemcpyAsync(bcd, bc, n, hipMemcpyHostToDevice, s));
+  }
+  int32_t *std_ = cell_status ? (int32_t *)h->stage[nd + 1] : nullptr;
+  if (std_) HIP_TRY(hipMemcpyAsync(std_, cell_status, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  int rc = step_dev_impl(h, grid, dt, D(0), D(1), Yd, bcd, Ypd, Yp2d, D(2), std_, stats, nullptr,
+                         nullptr, nullptr);
+  if (rc != NRG_CHEM_OK && rc != NRG_CHEM_E_STEP_ZERO && rc != NRG_CHEM_E_REPEATED_FAIL) return rc;
+  for (int q = 0; q < S; ++q) {
+    HIP_TRY(hipMemcpyAsync(Y_prod[q], Ypd[q], bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(Y_prod2[q], Yp2d[q], bytes, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipMemcpyAsync(E_f_prod, D(2), bytes, hipMemcpyDeviceToHost, s));
+  if (std_) HIP_TRY(hipMemcpyAsync(cell_status, std_, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return rc;
+}
+
+static int closure_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, int apply,
+                        const double *rho, const double *const *v, int32_t ndim,
+                        const double *E_f_prod, const double *const *Y_prod, const int8_t *bc,
+                        double *E_f, double *const *Y, double *T, double *p)
+{
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (!rho || !E_f || !Y || !T) return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL");
+  if (ndim < 0 || ndim > 3 || (ndim > 0 && !v)) return set_error(NRG_CHEM_E_ARG, "bad velocity");
+  if (apply && (!E_f_prod || !Y_prod || !(dt > 0.0)))
+    return set_error(NRG_CHEM_E_ARG, "sources and dt > 0 required");
+  ClosureArgs a{};
+  int st = make_grid(grid, a.g);
+  if (st) return st;
+  a.dt = dt;
+  a.apply_sources = apply;
+  a.ndim = ndim;
+  a.mech = h->dev_mech;
+  a.rho = rho;
+  for (int d = 0; d < ndim; ++d) {
+    if (!v[d]) return set_error(NRG_CHEM_E_ARG, "velocity component %d is NULL", d);
+    a.v[d] = v[d];
+  }
+  a.Efp = E_f_prod;
+  for (int s = 0; s < S; ++s) {
+    if (!Y[s] || (apply && !Y_prod[s])) return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", s + 1);
+    a.Y[s] = Y[s];
+    a.Yp[s] = apply ? Y_prod[s] : nullptr;
+  }
+  a.bc = bc;
+  a.E_f = E_f;
+  a.T = T;
+  a.p = p;
+  HIP_TRY(hipSetDevice(h->device));
+  if (a.g.ninner > 0) {
+    const long long nb = (a.g.ninner + 255) / 256;
+    hipLaunchKernelGGL(closure_kernel, dim3((unsigned)nb), dim3(256), 0, h->stream, a);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return NRG_CHEM_OK;
+}
+
+int nrg_chem_closure_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *rho,
+                         const double *const *v, int32_t ndim, const double *E_f_prod,
+                         const double *const *Y_prod, const int8_t *bc, double *E_f,
+                         double *const *Y, double *T, double *p)
+{
+  return closure_impl(h, grid, dt, 1, rho, v, ndim, E_f_prod, Y_prod, bc, E_f, Y, T, p);
+}
+
+int nrg_state_equation_dev(nrg_chem_t *h, const nrg_grid *grid, const double *rho,
+                           const double *const *v, int32_t ndim, const double *E_f,
+                           const double *const *Y, const int8_t *bc, double *T, double *p)
+{
+  return closure_impl(h, grid, 0.0, 0, rho, v, ndim, nullptr, nullptr, bc, const_cast<double *>(E_f),
+                      const_cast<double *const *>(Y), T, p);
+}
+
+int nrg_energy_from_temperature_dev(nrg_chem_t *h, const nrg_grid *grid, const double *T,
+                                    const double *const *Y, const double *const *v, int32_t ndim,
+                                    double *E_f)
+{
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (!T || !Y || !E_f) return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL");
+  if (ndim < 0 || ndim > 3 || (ndim > 0 && !v)) return set_error(NRG_CHEM_E_ARG, "bad velocity");
+  EnergyArgs a{};
+  int st = make_grid(grid, a.g);
+  if (st) return st;
+  a.ndim = ndim;
+  a.mech = h->dev_mech;
+  a.T = T;
+  for (int s = 0; s < S; ++s) {
+    if (!Y[s]) return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", s + 1);
+    a.Y[s] = Y[s];
+  }
+  for (int d = 0; d < ndim; ++d) a.v[d] = v[d];
+  a.E_f = E_f;
+  HIP_TRY(hipSetDevice(h->device));
+  if (a.g.ninner > 0) {
+    const long long nb = (a.g.ninner + 255) / 256;
+    hipLaunchKernelGGL(energy_kernel, dim3((unsigned)nb), dim3(256), 0, h->stream, a);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return NRG_CHEM_OK;
+}
+
+}  // extern "C"
