@@ -229,6 +229,8 @@ def load_library(path=None):
     lib.nrg_chem_step_dev.restype = C.c_int
     lib.nrg_chem_step.argtypes = step_args
     lib.nrg_chem_step.restype = C.c_int
+    lib.nrg_chem_step_dev_diag.argtypes = step_args + [vp, vp, vp]
+    lib.nrg_chem_step_dev_diag.restype = C.c_int
     lib.nrg_chem_closure_dev.argtypes = [vp, C.POINTER(NrgGrid), C.c_double, vp, vp, C.c_int32,
                                          vp, vp, vp, vp, vp, vp, vp]
     lib.nrg_chem_closure_dev.restype = C.c_int

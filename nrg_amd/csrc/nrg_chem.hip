@@ -536,11 +536,10 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     return set_error(NRG_CHEM_E_ARG, "jacobian mode %d not available", o.jacobian);
   if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64))
     return set_error(NRG_CHEM_E_ARG, "block_size must be 64, 128, 192 or 256");
-  int st = check_device(o.device);
-  if (st) return st;
   auto *h = new (std::nothrow) nrg_chem_handle();
   if (!h) return set_error(NRG_CHEM_E_ARG, "out of host memory");
-  st = build_mech(mech, h->host_mech);
+  int st = build_mech(mech, h->host_mech);
+  if (!st) st = check_device(o.device);
   if (st) {
     delete h;
     return st;

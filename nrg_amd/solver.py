@@ -149,12 +149,9 @@ class ChemistryStep:
             rc = self.lib.nrg_chem_step_dev(self.h, C.byref(grid), float(dt), T, p, rho, Yp, bc,
                                             Ypp, Yp2, E_f_prod, status, C.byref(st))
         else:
-            fn = self.lib.nrg_chem_step_dev_diag
-            fn.argtypes = [C.c_void_p, C.POINTER(capi.NrgGrid), C.c_double] + [C.c_void_p] * 10 + [
-                C.POINTER(NrgChemStats), C.c_void_p, C.c_void_p, C.c_void_p]
-            fn.restype = C.c_int
-            rc = fn(self.h, C.byref(grid), float(dt), T, p, rho, Yp, bc, Ypp, Yp2, E_f_prod,
-                    status, C.byref(st), *counters)
+            rc = self.lib.nrg_chem_step_dev_diag(self.h, C.byref(grid), float(dt), T, p, rho, Yp,
+                                                 bc, Ypp, Yp2, E_f_prod, status, C.byref(st),
+                                                 *counters)
         self.last_stats = st
         check(self.lib, rc)
         return st
