@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Throughput of NRG's operator-split chemistry step on one MI355X.
+
+Workload: a 256^3 H2-air field (KEROMNES, 9 species / 24 reactions) of seeded
+spherical hot kernels (T 1000-2600 K, radius 4-20 cells) in a 300-900 K fresh
+mixture, p ~ U[0.5, 5] atm, FP64 structure-of-arrays fields with NRG's ghost
+layers, dt = 1e-7 s.  One "step" is one split step: the chemistry step
+(nrg_chem_step_dev: per-cell Gear-BDF with SLATEC DDRIV3 semantics) followed by
+the legacy CABARET source application and e->T inversion (nrg_chem_closure_dev).
+Fields stay resident in HBM; the timed region holds exactly --steps split steps.
+
+Prints one JSON line (metric, value in cell-steps/s, roofline of the chemistry
+kernel measured with HIP events, CPU baseline from the oracle on a sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "chem cell-steps/s at 256^3: FP64 integrator vs MLP surrogate @≤1e-2 rel T err"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256, help="cells per dimension")
+    ap.add_argument("--dt", type=float, default=1.0e-7)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-sample", type=int, default=16384,
+                    help="cells of the workload timed on the host through the oracle (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device copy of this size before the timed region, as the "
+                         "known-size reference for HBM counters (FETCH_SIZE / WRITE_SIZE)")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU only (the chemistry step is cell-local; "
+              "see DESIGN.md for how it shards); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from nrg_amd import flop_model, inputs
+    from nrg_amd.mechanism import load_mechanism
+    from nrg_amd.solver import DataManager, chemical_kinetics_solver_c
+
+    mech = load_mechanism()
+    S = mech.species_number
+    n = args.n
+    shape = (n, n, n)
+    dev = torch.device("cuda:0")
+
+    # ---- state in HBM ---------------------------------------------------------------
+    st = inputs.hot_kernels_3d(mech, shape, seed=args.seed, device=dev)
+    dm = DataManager(shape, device=dev, mech=mech)
+    g = dm.grid
+    view = (n, n, n)
+    dm.fields["temperature"].inner(g).copy_(st["T"].view(view))
+    dm.fields["pressure"].inner(g).copy_(st["p"].view(view))
+    dm.fields["density"].inner(g).copy_(st["rho"].view(view))
+    Yf = dm.fields["specie_mass_fraction"]
+    for s in range(S):
+        Yf.pr[s].inner(g).copy_(st["Y"][s].view(view))
+    solver = chemical_kinetics_solver_c(dm)
+    E_f = dm.fields["full_energy"]
+    solver.engine.energy_from_temperature_dev(g, dm.fields["temperature"].ptr(), Yf.ptrs(),
+                                              E_f.ptr())
+    ncells = n * n * n
+
+    # CPU-baseline / audit sample: fixed random cells of the initial state
+    sample = None
+    if args.cpu_sample > 0:
+        rng = np.random.default_rng(args.seed)
+        idx = np.sort(rng.choice(ncells, size=min(args.cpu_sample, ncells), replace=False))
+        sample = dict(idx=idx, T=st["T"].cpu().numpy()[idx], rho=st["rho"].cpu().numpy()[idx],
+                      Y=st["Y"].cpu().numpy()[:, idx],
+                      E_f=E_f.inner(g).reshape(-1).cpu().numpy()[idx])
+    del st
+    torch.cuda.synchronize()
+
+    def split_step():
+        stats = solver.solve_chemical_kinetics(args.dt)
+        solver.apply_split_sources(args.dt)
+        return stats
+
+    # ---- warmup (the first warmup step is also the audit step) ---------------------------
+    audit_out = None
+    for w in range(args.warmup):
+        stats = split_step()
+        if w == 0 and sample is not None:
+            idx = sample["idx"]
+            audit_out = dict(
+                Y_prod=np.stack([solver.Y_prod.pr[s].inner(g).reshape(-1)[idx].cpu().numpy()
+                                 for s in range(S)]),
+                T=dm.fields["temperature"].inner(g).reshape(-1)[idx].cpu().numpy(),
+                Y=np.stack([Yf.pr[s].inner(g).reshape(-1)[idx].cpu().numpy() for s in range(S)]))
+    torch.cuda.synchronize()
+
+    if args.copy_calibration_mib > 0:
+        nel = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    # ---- timed region ------------------------------------------------------------------
+    kernel_ms, flops, nfe, nstep_max, integ = [], [], 0, 0, 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats = split_step()
+        kernel_ms.append(stats.kernel_ms)
+        flops.append(flop_model.step_flops(stats))
+        nfe += stats.nfe_sum
+        nstep_max = max(nstep_max, stats.nstep_max)
+        integ += stats.cells_integrated
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = ncells * args.steps / wall
+
+    mean_ms = float(np.mean(kernel_ms))
+    mean_flops = float(np.mean(flops))
+    achieved_tflops = mean_flops / (mean_ms * 1e-3) / 1e12
+    roofline = {
+        "kernel": "chem_step_kernel",
+        "bound": "mfma",
+        "compute": "fp64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+        "achieved": round(achieved_tflops, 4),
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tflops / PEAK_FP64_TFLOPS, 6),
+        "traffic": None,
+        "avg_kernel_ms": round(mean_ms, 3),
+        "algorithmic_flops_per_launch": mean_flops,
+        "hbm_boundary_GBs": round(flop_model.BYTES_BOUNDARY * ncells / (mean_ms * 1e-3) / 1e9, 2),
+    }
+
+    # ---- CPU baseline and audit on the sample (oracle = CPU restatement) ------------------
+    cpu_baseline = None
+    audit = None
+    if sample is not None:
+        from oracle import Oracle
+        orc = Oracle(mech)
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        m = len(sample["idx"])
+        c0 = time.perf_counter()
+        o = orc.chem_cells(args.dt, sample["T"], sample["rho"], sample["Y"], nthreads=threads)
+        E2, Y2, T2, _ = orc.closure_cells(args.dt, sample["rho"], o["E_f_prod"], o["Y_prod"],
+                                          sample["E_f"], sample["Y"], sample["T"],
+                                          nthreads=threads)
+        c1 = time.perf_counter()
+        cpu_baseline = {
+            "value": round(m / (c1 - c0), 1),
+            "unit": "cell-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": "%d uniformly sampled cells of the initial 256^3 state, one split step "
+                      "(chemistry + closure), C oracle with OpenMP" % m,
+        }
+        if audit_out is not None:
+            dY = np.abs(audit_out["Y"] - Y2).max()
+            relT = float((np.abs(audit_out["T"] - T2) / T2).max())
+            audit = {"cells": m, "max_abs_Y": float(dY), "max_rel_T": relT,
+                     "bar_Y": 1e-9, "bar_T": 1e-10}
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded hot-kernel field)",
+        "config": {
+            "workload": "FP64 Gear-BDF split step (SLATEC DDRIV3 semantics, FD Jacobian), "
+                        "256^3 hot kernels in fresh H2-air, KEROMNES 9sp/24r, dt=1e-7 s",
+            "cells": ncells, "dt": args.dt, "mechanism": "KEROMNES",
+            "path": "fp64_integrator",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu_baseline,
+        "audit_vs_oracle": audit,
+        "work": {"nfe_per_step": nfe / args.steps, "integrated_cells_per_step": integ / args.steps,
+                 "nstep_max": nstep_max},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
