@@ -36,6 +36,15 @@ def _torch():
     return torch
 
 
+def _order_after_torch():
+    """The library launches on its own HIP stream; work torch queued on its
+    current stream (field fills, copies) must finish before the library reads
+    those fields.  Every library call returns with its own stream drained."""
+    torch = _torch()
+    if torch.cuda.is_available() and torch.cuda.is_initialized():
+        torch.cuda.current_stream().synchronize()
+
+
 class Field:
     """One FP64 (or int8) field with NRG's bounds: Fortran (lo:hi) per dim, column-major.
 
@@ -141,6 +150,7 @@ class ChemistryStep:
     def step_dev(self, grid, dt, T, rho, Y, Y_prod, Y_prod2, E_f_prod, bc=None, status=None,
                  counters=None, p=None):
         """Device-pointer step.  Fields are ints (device addresses); Y* are lists of addresses."""
+        _order_after_torch()
         st = NrgChemStats()
         Yp = capi.ptr_array(Y)
         Ypp = capi.ptr_array(Y_prod)
@@ -157,6 +167,7 @@ class ChemistryStep:
         return st
 
     def closure_dev(self, grid, dt, rho, E_f_prod, Y_prod, E_f, Y, T, p=None, bc=None, v=None):
+        _order_after_torch()
         vv = capi.ptr_array(v) if v else None
         nd = len(v) if v else 0
         rc = self.lib.nrg_chem_closure_dev(self.h, C.byref(grid), float(dt), rho, vv, nd, E_f_prod,
@@ -164,6 +175,7 @@ class ChemistryStep:
         check(self.lib, rc)
 
     def state_equation_dev(self, grid, rho, E_f, Y, T, p=None, bc=None, v=None):
+        _order_after_torch()
         vv = capi.ptr_array(v) if v else None
         nd = len(v) if v else 0
         rc = self.lib.nrg_state_equation_dev(self.h, C.byref(grid), rho, vv, nd, E_f,
@@ -171,6 +183,7 @@ class ChemistryStep:
         check(self.lib, rc)
 
     def energy_from_temperature_dev(self, grid, T, Y, E_f, v=None):
+        _order_after_torch()
         vv = capi.ptr_array(v) if v else None
         nd = len(v) if v else 0
         rc = self.lib.nrg_energy_from_temperature_dev(self.h, C.byref(grid), T, capi.ptr_array(Y),
