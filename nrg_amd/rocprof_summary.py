@@ -1,0 +1,84 @@
+"""Summarise rocprofv3 CSV output for the chemistry kernels.
+
+    python -m nrg_amd.rocprof_summary stats   <kernel_stats.csv>
+    python -m nrg_amd.rocprof_summary counter <counter_collection.csv> [--calib-mib M]
+
+``stats`` prints calls / average / total duration per kernel.  ``counter``
+prints, per kernel, the average per-dispatch value of each collected counter
+(FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  With ``--calib-mib`` the
+largest device copy kernel (the known-size copy bench.py runs with
+--copy-calibration-mib) is taken as the reference: its expected bytes divided
+by its measured bytes gives the factor applied to the other kernels.
+"""
+import csv
+import sys
+from collections import defaultdict
+
+
+def _short(name, width=70):
+    return name if len(name) <= width else name[: width - 3] + "..."
+
+
+def stats(path):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    print("%-72s %6s %14s %14s %7s" % ("kernel", "calls", "avg_ms", "total_ms", "pct"))
+    for r in rows[:15]:
+        print("%-72s %6s %14.3f %14.3f %7.2f" % (_short(r["Name"]), r["Calls"],
+                                                float(r["AverageNs"]) / 1e6,
+                                                float(r["TotalDurationNs"]) / 1e6,
+                                                float(r["Percentage"])))
+
+
+def counters(path, calib_mib=None):
+    rows = list(csv.DictReader(open(path)))
+    name_key = "Kernel_Name" if "Kernel_Name" in rows[0] else "Kernel-Name"
+    acc = defaultdict(lambda: defaultdict(float))
+    disp = defaultdict(set)
+    for r in rows:
+        k = r[name_key]
+        cname = r.get("Counter_Name") or r.get("Counter-Name")
+        acc[k][cname] += float(r.get("Counter_Value") or r.get("Counter-Value"))
+        disp[k].add(r.get("Dispatch_Id") or r.get("Dispatch-Id"))
+    factor = {}
+    if calib_mib:
+        copies = [k for k in acc if "copy" in k.lower() or "elementwise" in k.lower()]
+        best = None
+        for k in copies:
+            for c, v in acc[k].items():
+                per = v / max(1, len(disp[k]))
+                if best is None or per > best[2]:
+                    best = (k, c, per)
+        if best:
+            expected_kib = calib_mib * 1024.0
+            for c in ("FETCH_SIZE", "WRITE_SIZE"):
+                factor[c] = expected_kib / best[2]
+            print("calibration kernel: %s  measured %.1f KiB/dispatch, expected %.1f KiB -> "
+                  "factor %.4f" % (_short(best[0]), best[2], expected_kib, expected_kib / best[2]))
+    print("%-72s %6s %18s %18s" % ("kernel", "disp", "counter", "avg/dispatch"))
+    for k in sorted(acc, key=lambda k: -sum(acc[k].values())):
+        nd = max(1, len(disp[k]))
+        for c, v in sorted(acc[k].items()):
+            f = factor.get(c, 1.0)
+            print("%-72s %6d %18s %18.1f" % (_short(k), nd, c, v / nd * f))
+
+
+def main(argv):
+    if len(argv) < 2:
+        print(__doc__)
+        return 2
+    if argv[0] == "stats":
+        stats(argv[1])
+    elif argv[0] == "counter":
+        calib = None
+        if "--calib-mib" in argv:
+            calib = float(argv[argv.index("--calib-mib") + 1])
+        counters(argv[1], calib)
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
