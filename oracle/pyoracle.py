@@ -45,6 +45,9 @@ class BdfOut(C.Structure):
 
 
 def _ptrs(arrs):
+    for a in arrs:
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("field rows handed to the oracle must be C-contiguous")
     return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
 
 
@@ -199,7 +202,8 @@ class Oracle:
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         Y = np.ascontiguousarray(Y, dtype=np.float64)
         yp = np.zeros((self.S, n))
-        yp2 = np.zeros((self.S, n)) if Y_prod2 is None else np.array(Y_prod2, dtype=np.float64)
+        yp2 = (np.zeros((self.S, n)) if Y_prod2 is None
+               else np.array(Y_prod2, dtype=np.float64, order="C"))
         ef = np.zeros(n)
         st = np.zeros(n, dtype=np.int32)
         ns = np.zeros(n, dtype=np.int32)
@@ -219,9 +223,9 @@ class Oracle:
     def closure_cells(self, dt, rho, E_f_prod, Y_prod, E_f, Y, T, bc=None, nthreads=1):
         """Split closure + e->T.  Returns (E_f, Y, T, p) new arrays."""
         n = len(T)
-        E_f = np.array(E_f, dtype=np.float64)
-        Y = np.array(Y, dtype=np.float64)
-        T = np.array(T, dtype=np.float64)
+        E_f = np.array(E_f, dtype=np.float64, order="C")
+        Y = np.array(Y, dtype=np.float64, order="C")
+        T = np.array(T, dtype=np.float64, order="C")
         p = np.zeros(n)
         Y_prod = np.ascontiguousarray(Y_prod, dtype=np.float64)
         bcp = None
