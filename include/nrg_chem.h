@@ -66,6 +66,12 @@ enum {
   NRG_JAC_ANALYTIC = 1         /* analytic d(omega)/dC at fixed k: fast mode                  */
 };
 
+/* Launch form of the per-cell integrator (same arithmetic, same results). */
+enum {
+  NRG_FORM_PLAIN = 0,          /* one cell per lane, one launch over all inner cells          */
+  NRG_FORM_PERSISTENT = 1      /* resident grid; lanes refill from a work queue as cells end  */
+};
+
 /*
  * Mechanism and thermochemistry, exactly the arrays NRG's
  * chemical_properties (package_library/src/chemical_properties_class.f90:15-52)
@@ -108,8 +114,11 @@ typedef struct nrg_chem_opts {
   int32_t mxord;
   int32_t jacobian;            /* NRG_JAC_*                                                    */
   int32_t device;              /* HIP device ordinal                                           */
-  int32_t block_size;          /* threads per block of the per-cell kernel; 0 = default        */
-  int32_t reserved[7];
+  int32_t form;                /* NRG_FORM_*                                                   */
+  int32_t jac_batch_pct;       /* a wave runs a batched Jacobian when at least this percentage
+                                  of its pending lanes wait for one (1..100; 0 = default 50)   */
+  int32_t waves_per_cu;        /* persistent form: resident waves per CU (0 = occupancy limit) */
+  int32_t reserved[5];
 } nrg_chem_opts;
 
 /*

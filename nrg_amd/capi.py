@@ -33,6 +33,9 @@ NRG_CELL_SINGULAR = 5
 NRG_JAC_FD_SLATEC = 0
 NRG_JAC_ANALYTIC = 1
 
+NRG_FORM_PLAIN = 0
+NRG_FORM_PERSISTENT = 1
+
 
 class NrgChemMech(C.Structure):
     _fields_ = [
@@ -63,8 +66,10 @@ class NrgChemOpts(C.Structure):
         ("mxord", C.c_int32),
         ("jacobian", C.c_int32),
         ("device", C.c_int32),
-        ("block_size", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("form", C.c_int32),
+        ("jac_batch_pct", C.c_int32),
+        ("waves_per_cu", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -119,7 +124,9 @@ def default_opts(**kw):
     o.mxord = 5
     o.jacobian = NRG_JAC_FD_SLATEC
     o.device = 0
-    o.block_size = 0
+    o.form = NRG_FORM_PLAIN
+    o.jac_batch_pct = 0
+    o.waves_per_cu = 0
     for k, v in kw.items():
         setattr(o, k, v)
     return o

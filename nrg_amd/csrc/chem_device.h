@@ -1,19 +1,25 @@
 // chem_device.h -- device code of the gfx950 chemistry step.
 //
-// One cell per lane.  The cell's whole integrator state (Nordsieck history,
-// iteration matrix, rate constants) lives in VGPRs/AGPRs: every array index
-// below is a compile-time constant after unrolling, and the few order- or
-// pivot-dependent accesses are written as select chains.  Mechanism numeric
-// coefficients are read from an LDS copy (MechCoef) shared by the block.
+// One cell per lane.  A cell's Gear-BDF state (Nordsieck history, iteration
+// matrix, work vectors) lives in VGPRs/AGPRs; its 2R rate constants live in a
+// per-lane LDS slot; the mechanism's numeric coefficients are one LDS copy per
+// block.  Every register-array index is a compile-time constant after
+// unrolling; the few order- or pivot-dependent accesses are select chains.
 //
-// Arithmetic follows the reference's evaluation order operation by operation
-// (this translation unit is compiled with -ffp-contract=off), so results agree
-// with NRG's Fortran step to rounding of the transcendental functions.
+// SLATEC's DDRIV3 is restated as a reverse-communication state machine
+// (advance()): a lane runs until it needs a right-hand side f(y) or a
+// Jacobian, then the wave evaluates f for all lanes that asked at one code
+// site, or -- when enough lanes wait for one -- runs a batched
+// finite-difference Jacobian + LU for those lanes (jac_batch()).  Decisions
+// and arithmetic follow the reference operation by operation (this TU is
+// built with -ffp-contract=off), so results agree with NRG's Fortran step to
+// the rounding of the transcendental functions.
+//
 // Reference: computing_module/src/current_build/chemical_kinetics_solver.f90
-// (rates :360-493, RHS :495-540, cell step :228-346) and SLATEC SDRIVE
-// (computing_module/src/third_party/slatec/ddriv3.f90, ddstp.f90, ddntl.f90,
-// ddpst.f90, ddcor.f90, ddpsc.f90, ddscl.f90, ddntp.f90, dgefa.f90, dgesl.f90,
-// dnrm2.f90).
+// (rates :360-493, RHS :495-540, cell step :228-346) and SLATEC SDRIVE in
+// computing_module/src/third_party/slatec/ (ddriv3.f90, ddstp.f90, ddntl.f90,
+// ddpst.f90, ddcor.f90, ddpsc.f90, ddscl.f90, ddntp.f90, dgefa.f90,
+// dgesl.f90, dnrm2.f90).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,6 +33,7 @@ namespace nrg {
 constexpr int S = topo::S;
 constexpr int R = topo::R;
 constexpr int MAXORD = 5;
+constexpr int WAVE = 64;
 
 constexpr double R_GAS_J = 8.31446261815324;   // package_library/src/global_data.f90:36
 constexpr double T_REF = 298.15;               // :38
@@ -61,6 +68,57 @@ struct BdfConst {
 
 NRG_DI double dmax(double a, double b) { return (a >= b) ? a : b; }
 NRG_DI double dmin(double a, double b) { return (a <= b) ? a : b; }
+
+// A value pinned in a VGPR.  Select chains over register arrays go through it,
+// so the optimiser cannot turn "select between loads" into one load through a
+// selected address (which would keep the whole array in private memory).
+NRG_DI double opaque(double x)
+{
+  asm("" : "+v"(x));
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// Compile-time facts about the topology
+// ---------------------------------------------------------------------------
+// Does fun (chemical_kinetics_solver.f90:509-527) multiply by [M] for reaction r?
+constexpr bool m_in_rhs(int r)
+{
+  for (int c = 0; c < 3; ++c)
+    if (c < topo::nl[r] && topo::left[r][c] == S) return true;
+  for (int c = 0; c < 3; ++c)
+    if (c < topo::nr[r] && topo::right[r][c] == S) return true;
+  return false;
+}
+constexpr int m_slot(int r)
+{
+  int k = 0;
+  for (int q = 0; q < r; ++q) k += m_in_rhs(q) ? 1 : 0;
+  return m_in_rhs(r) ? k : -1;
+}
+struct MSlotTable {
+  int v[R];
+};
+constexpr MSlotTable make_mslot()
+{
+  MSlotTable t{};
+  for (int r = 0; r < R; ++r) t.v[r] = m_slot(r);
+  return t;
+}
+constexpr MSlotTable kMSlot = make_mslot();   // [M] register slot of reaction r, or -1
+constexpr int count_m()
+{
+  int k = 0;
+  for (int r = 0; r < R; ++r) k += m_in_rhs(r) ? 1 : 0;
+  return k;
+}
+constexpr int NM = count_m() > 0 ? count_m() : 1;
+// [M] needed at setup: third-body and fall-off reactions.
+constexpr bool uses_m(int r)
+{
+  return topo::rtype[r] == 1 || topo::rtype[r] == 2 || topo::rtype[r] == 3 ||
+         topo::rtype[r] == 6 || topo::rtype[r] == 7;
+}
 
 // ---------------------------------------------------------------------------
 // NASA-7 thermochemistry (thermophysical_properties_class.f90:225-300)
@@ -112,21 +170,12 @@ NRG_DI double e_molar(const MechCoef &mc, double T, int s)
 }
 
 // ---------------------------------------------------------------------------
-// Kinetics: rate constants (chemical_kinetics_solver.f90:360-493)
+// Rate constants (chemical_kinetics_solver.f90:360-493) into the lane's LDS
+// slot: kl[r*WAVE] = k_f(r), kl[(R+r)*WAVE] = k_r(r).  [M] of the reactions
+// whose RHS term uses it go to Mc (:235-248).
 // ---------------------------------------------------------------------------
-struct Rates {
-  double kf[R], kr[R], Mc[R];
-};
-
-// [M]_r for the reactions that use it (:235-248): third-body reactions read it
-// in the RHS, fall-off reactions in the reduced pressure.
-NRG_DI constexpr bool uses_m(int r)
-{
-  return topo::rtype[r] == 1 || topo::rtype[r] == 2 || topo::rtype[r] == 3 ||
-         topo::rtype[r] == 6 || topo::rtype[r] == 7;
-}
-
-NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], Rates &k)
+NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], double *kl,
+                           double (&Mc)[NM])
 {
   const double T = dmin(Tin, 10000.0);
   double sv[S], hv[S];
@@ -137,13 +186,11 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
+    double m = 0.0;
     if (uses_m(r)) {
-      double m = 0.0;
 #pragma unroll
       for (int s = 0; s < S; ++s) m = m + C[s] * mc.eff[r][s];
-      k.Mc[r] = m;
-    } else {
-      k.Mc[r] = 0.0;
+      if (kMSlot.v[r] >= 0) Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] = m;
     }
     double d_s = 0.0, d_h = 0.0;
 #pragma unroll
@@ -170,14 +217,14 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
     } else if (rt == 2 || rt == 7) {
       const double k0 = (mc.A_low[r] * pow(T, mc.beta_low[r])) * exp(mc.mElow_R[r] / T);
       const double kinf = (mc.A[r] * pow(T, mc.beta[r])) * exp(mc.mE_R[r] / T);
-      const double pr = (k0 * k.Mc[r]) / kinf;
+      const double pr = (k0 * m) / kinf;
       kf = kinf * (pr / (1.0 + pr));
     } else if (rt == 3) {
       double k0 = exp(mc.mElow_R[r] / T);
       double kinf = exp(mc.mE_R[r] / T);
       k0 = (mc.A_low[r] * pow(T, mc.beta_low[r])) * k0;
       kinf = (mc.A[r] * pow(T, mc.beta[r])) * kinf;
-      const double pr = (k0 * k.Mc[r]) / kinf;
+      const double pr = (k0 * m) / kinf;
       const double fcent = (mc.one_m_alpha[r] * exp(-(T / mc.troe[1][r])) +
                             mc.troe[0][r] * exp(-(T / mc.troe[2][r]))) +
                            exp(-(mc.troe[3][r] / T));
@@ -197,35 +244,37 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
     }
     if (rt == 0 || rt == 1 || rt == 2 || rt == 3) {
       const double Kp = exp(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
-      // pow(x, d_nu) with the integral d_nu of this reaction; d_nu = 0 gives 1 exactly.
+      // K_c = K_p * ((P_atm/R_u)/T)**d_nu; d_nu = 0 gives pow(x, 0) = 1 exactly
       const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * pow(mc.patm_r / T, d_nu);
       kr = kf / Kc;
     }
-    k.kf[r] = kf;
-    k.kr[r] = kr;
+    kl[r * WAVE] = kf;
+    kl[(R + r) * WAVE] = kr;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Mass-action right-hand side, exactly the accumulation order of fun (:507-536)
+// Mass-action right-hand side, the accumulation order of fun (:507-536).
 // ---------------------------------------------------------------------------
-NRG_DI void rhs(const Rates &k, const double (&Z)[S], double (&Zd)[S])
+NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[S], double (&Zd)[S])
 {
 #pragma unroll
   for (int s = 0; s < S; ++s) Zd[s] = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    double fwd = k.kf[r];
+    double fwd = kl[r * WAVE];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int sp = topo::left[r][c];
-      if (c < topo::nl[r]) fwd = fwd * ((sp == S) ? k.Mc[r] : Z[sp < S ? sp : 0]);
+      if (c < topo::nl[r])
+        fwd = fwd * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[sp < S ? sp : 0]);
     }
-    double rev = k.kr[r];
+    double rev = kl[(R + r) * WAVE];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int sp = topo::right[r][c];
-      if (c < topo::nr[r]) rev = rev * ((sp == S) ? k.Mc[r] : Z[sp < S ? sp : 0]);
+      if (c < topo::nr[r])
+        rev = rev * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[sp < S ? sp : 0]);
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -399,16 +448,34 @@ NRG_DI void gesl(const double (&a)[S][S], const int (&ipvt)[S], double (&b)[S])
 }
 
 // ---------------------------------------------------------------------------
-// Gear BDF state of one cell (WORK/IWORK of DDRIV3, private to the lane).
+// Gear BDF state of one cell (DDRIV3's WORK/IWORK, private to the lane) and
+// the program counter of the reverse-communication form.
 // ---------------------------------------------------------------------------
+enum Pc : int {
+  PC_260 = 0,      // DDRIV3 label 260: weights and guards, then DDSTP entry
+  PC_NTL_F,        // DDNTL after its f(y)
+  PC_ATTEMPT,      // DDSTP label 100: predict, then f(y_pred)
+  PC_PRED_F,       // after f(y_pred) (label 110)
+  PC_JAC_DONE,     // after the batched FD Jacobian + LU (DDPST)
+  PC_CORR,         // label 130: corrector (DDCOR)
+  PC_CORR_F,       // after f(y) of a corrector iteration
+  PC_FAIL_CONV,    // label 160: corrector failure
+  PC_ERRTEST,      // label 170: error test, accept, step/order selection
+  PC_DONE
+};
+enum Req : int { REQ_NONE = 0, REQ_F = 1, REQ_JAC = 2, REQ_DONE = 3 };
+enum CellOutcome { OUT_OK = 0, OUT_MXSTEP = 2, OUT_STEP_ZERO = 3, OUT_REPEATED_FAIL = 4,
+                   OUT_SINGULAR = 5, OUT_EPS = 6 };
+
 struct Cell {
   double yh[MAXORD + 1][S];   // yh[j][i] = YH(i, j+1)
   double y[S], ywt[S], save1[S], save2[S], fac[S];
-  double P[S][S];             // iteration matrix / LU, column-major
+  double P[S][S];             // iteration matrix / LU factors, column-major
   int ipvt[S];
-  double t, h, rc, rmax, trend;
+  double t, h, rc, rmax, trend, told, d1, ctest;
   int nq, nwait, jtask, jstepl, nstep, nfe, nje, ncorr, natt;
-  bool convrg, ier;
+  int ntry, nfail, iter, pc, req, outcome;
+  bool convrg, ier, evaljc, ntl_restart;
 };
 
 NRG_DI double el_of(const MechCoef &mc, int j, int nq) { return mc.el[nq - 1][j - 1]; }
@@ -440,14 +507,14 @@ NRG_DI void psc(int nq, double (&yh)[MAXORD + 1][S])
   }
 }
 
-// YH(:, NQ+1) read through a select chain.
+// YH(:, col+1) read through a select chain (col in 1..MAXORD).
 NRG_DI void yh_col(const Cell &c, int col, double (&out)[S])
 {
 #pragma unroll
   for (int i = 0; i < S; ++i) {
-    double v = c.yh[1][i];
+    double v = opaque(c.yh[1][i]);
 #pragma unroll
-    for (int j = 2; j <= MAXORD; ++j) v = (col == j) ? c.yh[j][i] : v;
+    for (int j = 2; j <= MAXORD; ++j) v = (col == j) ? opaque(c.yh[j][i]) : v;
     out[i] = v;
   }
 }
@@ -472,95 +539,6 @@ NRG_DI void scl(Cell &c, double rh, const BdfConst &k)
   c.rc = c.rc * rh;
 }
 
-template <class F>
-NRG_DI void f_eval(Cell &c, const F &f, const double (&y)[S], double (&out)[S])
-{
-  f(y, out);
-}
-
-// DDNTL, JTASK >= 0 (slatec/ddntl.f90:52-63,152-161).  EL/TQ are precomputed.
-template <class F>
-NRG_DI void ntl(Cell &c, const F &f, const BdfConst &k)
-{
-  c.ier = false;
-  if (c.jtask == 0) c.rmax = 10000.0;
-  c.rc = 0.0;
-  c.convrg = false;
-  c.trend = 1.0;
-  c.nq = 1;
-  c.nwait = 3;
-  f(c.y, c.save2);
-  c.nfe++;
-#pragma unroll
-  for (int i = 0; i < S; ++i) c.save1[i] = c.save2[i] / dmax(1.0, c.ywt[i]);
-  const double sum = dnrm2(c.save1) / k.sqrtn;
-  if (sum > k.eps / fabs(c.h)) c.h = copysign(k.eps / sum, c.h);
-#pragma unroll
-  for (int i = 0; i < S; ++i) c.yh[1][i] = c.h * c.save2[i];
-#pragma unroll
-  for (int i = 0; i < S; ++i) c.fac[i] = k.sqrt_uround;
-}
-
-// DDPST, MITER = 2 (slatec/ddpst.f90:39-114,142-143): finite-difference
-// Jacobian with adaptive increments FAC, P = I - h*l0*J, LU factorisation.
-template <class F>
-NRG_DI void pst(Cell &c, const F &f, const BdfConst &k, const MechCoef &mc)
-{
-  c.nje++;
-  c.ier = false;
-  const double l0 = el_of(mc, 1, c.nq);
-#pragma unroll
-  for (int j = 0; j < S; ++j) {
-    const double ys = dmax(fabs(c.ywt[j]), fabs(c.y[j]));
-    double fj = c.fac[j];
-    double dy = fj * ys;
-    while (dy == 0.0) {
-      if (fj < 0.5) {
-        fj = dmin(100.0 * fj, 0.5);
-        dy = fj * ys;
-      } else {
-        dy = ys;
-        break;
-      }
-    }
-    dy = (c.nq == 1) ? copysign(dy, c.save2[j]) : copysign(dy, c.yh[2][j]);
-    dy = (c.y[j] + dy) - c.y[j];
-    const double yj = c.y[j];
-    c.y[j] = c.y[j] + dy;
-    double f1[S];
-    f(c.y, f1);
-    c.y[j] = yj;
-    const double factor = ((-l0) * c.h) / dy;
-#pragma unroll
-    for (int i = 0; i < S; ++i) c.P[j][i] = (f1[i] - c.save2[i]) * factor;
-    double diff = fabs(c.save2[0] - f1[0]);
-    double s2m = c.save2[0], s1m = f1[0];
-#pragma unroll
-    for (int i = 1; i < S; ++i) {
-      const double di = fabs(c.save2[i] - f1[i]);
-      if (di > diff) {
-        diff = di;
-        s2m = c.save2[i];
-        s1m = f1[i];
-      }
-    }
-    if (dmin(fabs(s2m), fabs(s1m)) > 0.0) {
-      const double scale = dmax(fabs(s2m), fabs(s1m));
-      if (diff > 0.5 * scale)
-        fj = dmax(k.facmin, fj * 0.5);
-      else if (k.br * scale <= diff && diff <= k.bl * scale)
-        fj = dmin(fj * 2.0, 0.5);
-      else if (diff < k.br * scale)
-        fj = dmin(k.bp * fj, 0.5);
-    }
-    c.fac[j] = fj;
-  }
-  c.nfe += S;
-#pragma unroll
-  for (int i = 0; i < S; ++i) c.P[i][i] = c.P[i][i] + 1.0;
-  if (gefa(c.P, c.ipvt) != 0) c.ier = true;
-}
-
 // DDCOR, MITER = 2, IMPL = 0, IERROR = 3 (slatec/ddcor.f90:51-54,98-108).
 NRG_DI double cor(Cell &c, const BdfConst &k)
 {
@@ -576,242 +554,60 @@ NRG_DI double cor(Cell &c, const BdfConst &k)
   return dnrm2(c.save2) / k.sqrtn;
 }
 
-// DDSTP, MINT = 2, no method switching (slatec/ddstp.f90:91-437).  Returns JSTATE.
-template <class F>
-NRG_DI int stp(Cell &c, const F &f, const BdfConst &k, const MechCoef &mc)
-{
-  const double told = c.t;
-  int ntry = 0, nfail = 0;
-  if (c.jtask <= 0) {
-    ntl(c, f, k);
-    if (c.h == 0.0) return 2;
-    if (c.ier) return 4;
-  }
-  for (;;) {                                               // label 100
-    ntry++;
-    if (ntry > 50) return 3;
-    c.natt++;
-    c.t = c.t + c.h;
-    psc<true>(c.nq, c.yh);
-    bool evaljc = (fabs(c.rc - 1.0) > 0.3) || (c.nstep >= c.jstepl + 10);
-    bool converged = false;
-    int iter = 0;
-    double ctest = 0.0;
-    for (;;) {                                             // label 110
-      iter = 0;
-#pragma unroll
-      for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
-      f(c.y, c.save2);
-      c.nfe++;
-      if (evaljc || c.ier) {
-        pst(c, f, k, mc);
-        if (c.ier) break;                                  // -> 160
-        c.convrg = false;
-        c.rc = 1.0;
-        c.jstepl = c.nstep;
-      }
-#pragma unroll
-      for (int i = 0; i < S; ++i) c.save1[i] = 0.0;
-      double d1 = 0.0;
-      for (;;) {                                           // label 130
-        const double d = cor(c, k);
-        if (iter > 0) c.trend = dmax(0.9 * c.trend, d / d1);
-        d1 = d;
-        ctest = dmin(2.0 * c.trend, 1.0) * d;
-        if (ctest <= k.eps) {
-          converged = true;
-          break;
-        }
-        iter++;
-        if (iter >= 3) break;
-        const double l0 = el_of(mc, 1, c.nq);
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i] + l0 * c.save1[i];
-        f(c.y, c.save2);
-        c.nfe++;
-      }
-      if (converged) break;
-      if (c.convrg) {                                      // stale Jacobian: refresh, retry
-        evaljc = true;
-        continue;
-      }
-      break;
-    }
-    if (!converged) {                                      // label 160
-      c.t = told;
-      psc<false>(c.nq, c.yh);
-      c.nwait = c.nq + 2;
-      if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
-      const double rh = (iter == 0) ? 0.3 : 0.9 * pow(k.eps / ctest, 0.2);
-      if (rh * c.h == 0.0) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
-        return 2;
-      }
-      scl(c, rh, k);
-      continue;
-    }
-    // label 170: error test
-    c.convrg = true;
-#pragma unroll
-    for (int i = 0; i < S; ++i) c.save2[i] = c.save1[i] / dmax(fabs(c.y[i]), c.ywt[i]);
-    const double etest = dnrm2(c.save2) / (tq_of(mc, 2, c.nq) * k.sqrtn);
-    if (etest > k.eps) {
-      c.t = told;
-      psc<false>(c.nq, c.yh);
-      nfail++;
-      if (nfail < 3 || c.nq == 1) {
-        if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
-        const double rh2 = 1.0 / (1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
-        double rh = rh2;
-        if (c.nq > 1) {
-          double col[S];
-          yh_col(c, c.nq, col);
-#pragma unroll
-          for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
-          const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
-          const double rh1 = 1.0 / dmax(1.0, 1.3 * pow(erdn / k.eps, 1.0 / (double)c.nq));
-          if (rh2 < rh1) {
-            c.nq = c.nq - 1;
-            c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
-            rh = rh1;
-          }
-        }
-        c.nwait = c.nq + 2;
-        if (rh * c.h == 0.0) {
-#pragma unroll
-          for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
-          return 2;
-        }
-        scl(c, rh, k);
-        continue;
-      }
-      nfail = 0;
-      c.jtask = 2;
-#pragma unroll
-      for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
-      ntl(c, f, k);
-      c.rmax = 10.0;
-      if (c.h == 0.0) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
-        return 2;
-      }
-      if (c.ier) return 4;
-      continue;
-    }
-    // accepted step (slatec/ddstp.f90:291-300)
-    c.nstep++;
-#pragma unroll
-    for (int j = 1; j <= MAXORD + 1; ++j) {
-      if (j <= c.nq + 1) {
-        const double e = el_of(mc, j, c.nq);
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.yh[j - 1][i] = c.yh[j - 1][i] + e * c.save1[i];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
-    // step-size and order selection (slatec/ddstp.f90:360-432)
-    if (c.jtask == 0 || c.jtask == 2) {
-      const double rh = 1.0 / dmax(k.uround, 1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
-      if (rh > 1.0) scl(c, rh, k);
-    } else if (c.nwait > 1) {
-      c.nwait = c.nwait - 1;
-      if (c.nwait == 1 && c.nq < k.maxord) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.yh[MAXORD][i] = c.save1[i];
-      }
-    } else {
-      double rh1 = 0.0, rh3 = 0.0;
-      if (c.nq != 1) {
-        double col[S];
-        yh_col(c, c.nq, col);
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
-        const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
-        rh1 = 1.0 / dmax(k.uround, 1.3 * pow(erdn / k.eps, 1.0 / (double)c.nq));
-      }
-      const double rh2 = 1.0 / dmax(k.uround, 1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
-      if (c.nq != k.maxord) {
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-          c.save2[i] = (c.save1[i] - c.yh[MAXORD][i]) / dmax(fabs(c.y[i]), c.ywt[i]);
-        const double erup = dnrm2(c.save2) / (tq_of(mc, 3, c.nq) * k.sqrtn);
-        rh3 = 1.0 / dmax(k.uround, 1.4 * pow(erup / k.eps, 1.0 / (double)(c.nq + 2)));
-      }
-      double rh;
-      bool change = true;
-      if (rh1 > rh2 && rh1 >= rh3) {
-        rh = rh1;
-        if (rh <= 1.0) {
-          change = false;
-        } else {
-          c.nq = c.nq - 1;
-          c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
-        }
-      } else if (rh2 >= rh1 && rh2 >= rh3) {
-        rh = rh2;
-        if (rh <= 1.0) change = false;
-      } else {
-        rh = rh3;
-        if (rh <= 1.0) {
-          change = false;
-        } else {
-          const double e = el_of(mc, c.nq + 1, c.nq);
-          const double den = (double)(c.nq + 1);
-          const int dst = c.nq + 1;
-#pragma unroll
-          for (int j = 2; j <= MAXORD; ++j) {
-#pragma unroll
-            for (int i = 0; i < S; ++i) c.yh[j][i] = (dst == j) ? (c.save1[i] * e) / den : c.yh[j][i];
-          }
-          c.nq = c.nq + 1;
-          c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq - 1);
-        }
-      }
-      if (change) {
-        scl(c, rh, k);
-        c.rmax = 10.0;
-      }
-      c.nwait = c.nq + 2;
-    }
-    return 1;
-  }
-}
-
 // DDNTP, K = 0 (slatec/ddntp.f90:25-32).
-NRG_DI void ntp(const Cell &c, double tout, double (&y)[S])
+NRG_DI void ntp(Cell &c, double tout)
 {
   double col[S];
   yh_col(c, c.nq, col);
 #pragma unroll
-  for (int i = 0; i < S; ++i) y[i] = col[i];
+  for (int i = 0; i < S; ++i) c.y[i] = col[i];
   const double r = (tout - c.t) / c.h;
 #pragma unroll
   for (int j = MAXORD; j >= 1; --j) {
     const bool on = (j <= c.nq);
 #pragma unroll
-    for (int i = 0; i < S; ++i) y[i] = on ? c.yh[j - 1][i] + r * y[i] : y[i];
+    for (int i = 0; i < S; ++i) c.y[i] = on ? c.yh[j - 1][i] + r * c.y[i] : c.y[i];
   }
 }
 
-// Per-cell outcome of the integration.
-enum CellOutcome { OUT_OK = 0, OUT_MXSTEP = 2, OUT_STEP_ZERO = 3, OUT_REPEATED_FAIL = 4,
-                   OUT_SINGULAR = 5, OUT_EPS = 6 };
+NRG_DI void copy_yh0_to_y(Cell &c)
+{
+#pragma unroll
+  for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
+}
 
-// DDRIV3 with the chemistry solver's settings (NSTATE=1, NTASK=1, NROOT=0,
-// IERROR=3, MINT=2, MITER=2, IMPL=0) from t = 0 to tout, y in/out.
-// slatec/ddriv3.f90:930-982 (init), :1292-1341 (YWT, guards), :1351-1449 (loop).
-template <class F>
-NRG_DI int ddriv3(Cell &c, const F &f, double (&y)[S], double tout, const BdfConst &k,
-                  const MechCoef &mc)
+NRG_DI void finish(Cell &c, int outcome)
+{
+  c.outcome = outcome;
+  c.pc = PC_DONE;
+  c.req = REQ_DONE;
+}
+
+NRG_DI void request_f(Cell &c, int resume)
+{
+  c.pc = resume;
+  c.req = REQ_F;
+}
+
+// DDNTL up to its f call (slatec/ddntl.f90:52-63); EL/TQ are precomputed.
+NRG_DI void ntl_head(Cell &c)
+{
+  c.ier = false;
+  if (c.jtask == 0) c.rmax = 10000.0;
+  c.rc = 0.0;
+  c.convrg = false;
+  c.trend = 1.0;
+  c.nq = 1;
+  c.nwait = 3;
+}
+
+// DDRIV3 init for NSTATE = 1 (slatec/ddriv3.f90:930-982): y holds C_in.
+NRG_DI void bdf_init(Cell &c, double tout, const BdfConst &k)
 {
   c.t = 0.0;
   double h = (tout - c.t) * (1.0 - 4.0 * k.uround);
   h = copysign(dmin(fabs(h), k.hmax), h);
   c.h = h;
-  const double hsign = copysign(1.0, h);
   c.jtask = 0;
   c.nstep = c.nfe = c.nje = c.ncorr = c.natt = 0;
   c.jstepl = 0;
@@ -822,39 +618,382 @@ NRG_DI int ddriv3(Cell &c, const F &f, double (&y)[S], double tout, const BdfCon
   c.nq = 1;
   c.nwait = 0;
   c.convrg = false;
+  c.iter = 0;
+  c.ntry = 0;
+  c.nfail = 0;
 #pragma unroll
-  for (int i = 0; i < S; ++i) {
-    c.yh[0][i] = y[i];
-    c.y[i] = y[i];
-  }
-  if (c.t == tout) return OUT_OK;
+  for (int i = 0; i < S; ++i) c.yh[0][i] = c.y[i];
+  c.pc = PC_260;
+  c.req = REQ_NONE;
+  c.outcome = OUT_OK;
+}
+
+// Run the cell's program until it needs f(y) (req = REQ_F, y set), a
+// Jacobian (REQ_JAC) or is finished (REQ_DONE).  On REQ_F resumption the
+// caller has put f(y) into save2.  hsign is +1 throughout (tout > 0).
+NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
+{
   for (;;) {
+    switch (c.pc) {
+      case PC_260: {
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.ywt[i] = dmax(k.ewt, fabs(c.y[i]));
+        for (int i = 0; i < S; ++i) c.ywt[i] = dmax(k.ewt, fabs(c.y[i]));
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.save2[i] = c.y[i] / c.ywt[i];
-    double sum = dnrm2(c.save2) / k.sqrtn;
-    sum = dmax(1.0, sum);
-    if (k.eps < sum * k.uround) {
+        for (int i = 0; i < S; ++i) c.save2[i] = c.y[i] / c.ywt[i];
+        double sum = dnrm2(c.save2) / k.sqrtn;
+        sum = dmax(1.0, sum);
+        if (k.eps < sum * k.uround) {
+          copy_yh0_to_y(c);
+          finish(c, OUT_EPS);
+          return;
+        }
+        if (c.nstep == k.mxstep) {
+          copy_yh0_to_y(c);
+          finish(c, OUT_MXSTEP);
+          return;
+        }
+        c.told = c.t;                                    // DDSTP entry
+        c.ntry = 0;
+        c.nfail = 0;
+        if (c.jtask <= 0) {
+          ntl_head(c);
+          c.ntl_restart = false;
+          request_f(c, PC_NTL_F);
+          return;
+        }
+        c.pc = PC_ATTEMPT;
+        break;
+      }
+      case PC_NTL_F: {                                   // rest of DDNTL (:68,152-161)
+        c.nfe++;
 #pragma unroll
-      for (int i = 0; i < S; ++i) y[i] = c.yh[0][i];
-      return OUT_EPS;
+        for (int i = 0; i < S; ++i) c.save1[i] = c.save2[i] / dmax(1.0, c.ywt[i]);
+        const double sum = dnrm2(c.save1) / k.sqrtn;
+        if (sum > k.eps / fabs(c.h)) c.h = copysign(k.eps / sum, c.h);
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.yh[1][i] = c.h * c.save2[i];
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.fac[i] = k.sqrt_uround;
+        if (c.ntl_restart) c.rmax = 10.0;
+        if (c.h == 0.0) {
+          copy_yh0_to_y(c);
+          finish(c, OUT_STEP_ZERO);
+          return;
+        }
+        c.pc = PC_ATTEMPT;
+        break;
+      }
+      case PC_ATTEMPT: {                                 // label 100
+        c.ntry++;
+        if (c.ntry > 50) {
+          finish(c, OUT_REPEATED_FAIL);
+          return;
+        }
+        c.natt++;
+        c.t = c.t + c.h;
+        psc<true>(c.nq, c.yh);
+        c.evaljc = (fabs(c.rc - 1.0) > 0.3) || (c.nstep >= c.jstepl + 10);
+        c.iter = 0;                                      // label 110
+        copy_yh0_to_y(c);
+        request_f(c, PC_PRED_F);
+        return;
+      }
+      case PC_PRED_F: {
+        c.nfe++;
+        if (c.evaljc || c.ier) {
+          c.pc = PC_JAC_DONE;
+          c.req = REQ_JAC;
+          return;
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.save1[i] = 0.0;
+        c.pc = PC_CORR;
+        break;
+      }
+      case PC_JAC_DONE: {
+        if (c.ier) {
+          c.pc = PC_FAIL_CONV;
+          break;
+        }
+        c.convrg = false;
+        c.rc = 1.0;
+        c.jstepl = c.nstep;
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.save1[i] = 0.0;
+        c.pc = PC_CORR;
+        break;
+      }
+      case PC_CORR_F: {
+        c.nfe++;
+        c.pc = PC_CORR;
+        break;
+      }
+      case PC_CORR: {                                    // label 130
+        const double d = cor(c, k);
+        if (c.iter > 0) c.trend = dmax(0.9 * c.trend, d / c.d1);
+        c.d1 = d;
+        c.ctest = dmin(2.0 * c.trend, 1.0) * d;
+        if (c.ctest <= k.eps) {
+          c.pc = PC_ERRTEST;
+          break;
+        }
+        c.iter++;
+        if (c.iter < 3) {
+          const double l0 = el_of(mc, 1, c.nq);
+#pragma unroll
+          for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i] + l0 * c.save1[i];
+          request_f(c, PC_CORR_F);
+          return;
+        }
+        if (c.convrg) {                                  // stale Jacobian: refresh (label 110)
+          c.evaljc = true;
+          c.iter = 0;
+          copy_yh0_to_y(c);
+          request_f(c, PC_PRED_F);
+          return;
+        }
+        c.pc = PC_FAIL_CONV;
+        break;
+      }
+      case PC_FAIL_CONV: {                               // label 160
+        c.t = c.told;
+        psc<false>(c.nq, c.yh);
+        c.nwait = c.nq + 2;
+        if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
+        const double rh = (c.iter == 0) ? 0.3 : 0.9 * pow(k.eps / c.ctest, 0.2);
+        if (rh * c.h == 0.0) {
+          copy_yh0_to_y(c);
+          finish(c, OUT_STEP_ZERO);
+          return;
+        }
+        scl(c, rh, k);
+        c.pc = PC_ATTEMPT;
+        break;
+      }
+      case PC_ERRTEST: {                                 // label 170
+        c.convrg = true;
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.save2[i] = c.save1[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+        const double etest = dnrm2(c.save2) / (tq_of(mc, 2, c.nq) * k.sqrtn);
+        if (etest > k.eps) {
+          c.t = c.told;
+          psc<false>(c.nq, c.yh);
+          c.nfail++;
+          if (c.nfail < 3 || c.nq == 1) {
+            if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
+            const double rh2 = 1.0 / (1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
+            double rh = rh2;
+            if (c.nq > 1) {
+              double col[S];
+              yh_col(c, c.nq, col);
+#pragma unroll
+              for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+              const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
+              const double rh1 = 1.0 / dmax(1.0, 1.3 * pow(erdn / k.eps, 1.0 / (double)c.nq));
+              if (rh2 < rh1) {
+                c.nq = c.nq - 1;
+                c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
+                rh = rh1;
+              }
+            }
+            c.nwait = c.nq + 2;
+            if (rh * c.h == 0.0) {
+              copy_yh0_to_y(c);
+              finish(c, OUT_STEP_ZERO);
+              return;
+            }
+            scl(c, rh, k);
+            c.pc = PC_ATTEMPT;
+            break;
+          }
+          c.nfail = 0;                                   // restart at order 1 (:275-288)
+          c.jtask = 2;
+          copy_yh0_to_y(c);
+          ntl_head(c);
+          c.ntl_restart = true;
+          request_f(c, PC_NTL_F);
+          return;
+        }
+        // accepted step (slatec/ddstp.f90:291-300)
+        c.nstep++;
+#pragma unroll
+        for (int j = 1; j <= MAXORD + 1; ++j) {
+          if (j <= c.nq + 1) {
+            const double e = el_of(mc, j, c.nq);
+#pragma unroll
+            for (int i = 0; i < S; ++i) c.yh[j - 1][i] = c.yh[j - 1][i] + e * c.save1[i];
+          }
+        }
+        copy_yh0_to_y(c);
+        // step-size and order selection (slatec/ddstp.f90:360-432)
+        if (c.jtask == 0 || c.jtask == 2) {
+          const double rh = 1.0 / dmax(k.uround, 1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
+          if (rh > 1.0) scl(c, rh, k);
+        } else if (c.nwait > 1) {
+          c.nwait = c.nwait - 1;
+          if (c.nwait == 1 && c.nq < k.maxord) {
+#pragma unroll
+            for (int i = 0; i < S; ++i) c.yh[MAXORD][i] = c.save1[i];
+          }
+        } else {
+          double rh1 = 0.0, rh3 = 0.0;
+          if (c.nq != 1) {
+            double col[S];
+            yh_col(c, c.nq, col);
+#pragma unroll
+            for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+            const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
+            rh1 = 1.0 / dmax(k.uround, 1.3 * pow(erdn / k.eps, 1.0 / (double)c.nq));
+          }
+          const double rh2 = 1.0 / dmax(k.uround, 1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
+          if (c.nq != k.maxord) {
+#pragma unroll
+            for (int i = 0; i < S; ++i)
+              c.save2[i] = (c.save1[i] - c.yh[MAXORD][i]) / dmax(fabs(c.y[i]), c.ywt[i]);
+            const double erup = dnrm2(c.save2) / (tq_of(mc, 3, c.nq) * k.sqrtn);
+            rh3 = 1.0 / dmax(k.uround, 1.4 * pow(erup / k.eps, 1.0 / (double)(c.nq + 2)));
+          }
+          double rh;
+          bool change = true;
+          if (rh1 > rh2 && rh1 >= rh3) {
+            rh = rh1;
+            if (rh <= 1.0) {
+              change = false;
+            } else {
+              c.nq = c.nq - 1;
+              c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
+            }
+          } else if (rh2 >= rh1 && rh2 >= rh3) {
+            rh = rh2;
+            if (rh <= 1.0) change = false;
+          } else {
+            rh = rh3;
+            if (rh <= 1.0) {
+              change = false;
+            } else {
+              const double e = el_of(mc, c.nq + 1, c.nq);
+              const double den = (double)(c.nq + 1);
+              const int dst = c.nq + 1;
+#pragma unroll
+              for (int j = 2; j <= MAXORD; ++j) {
+#pragma unroll
+                for (int i = 0; i < S; ++i)
+                  c.yh[j][i] = (dst == j) ? (c.save1[i] * e) / den : c.yh[j][i];
+              }
+              c.nq = c.nq + 1;
+              c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq - 1);
+            }
+          }
+          if (change) {
+            scl(c, rh, k);
+            c.rmax = 10.0;
+          }
+          c.nwait = c.nq + 2;
+        }
+        // DDSTP returned JSTATE = 1; DDRIV3 label 470 (:1371, NTASK = 1 test :1443-1449)
+        c.jtask = 1;
+        if (c.t < tout) {
+          c.pc = PC_260;
+          break;
+        }
+        ntp(c, tout);
+        finish(c, OUT_OK);
+        return;
+      }
+      default:
+        finish(c, c.outcome);
+        return;
     }
-    if (c.nstep == k.mxstep) {
+  }
+}
+
+// DDPST, MITER = 2 (slatec/ddpst.f90:39-114,142-143) for the lanes with
+// `mine` set: finite-difference Jacobian with adaptive increments FAC,
+// P = I - h*l0*J and its LU.  The column loop is wave-uniform, so the
+// per-column register selections are scalar branches, and one RHS copy
+// serves all nine columns.
+NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc, const double *kl,
+                      const double (&Mc)[NM])
+{
+  if (mine) {
+    c.nje++;
+    c.ier = false;
+  }
+  const double l0 = mine ? el_of(mc, 1, c.nq) : 0.0;
+#pragma nounroll
+  for (int j = 0; j < S; ++j) {
+    double ywj = 0.0, yj = 0.0, fj = 0.0, s2j = 0.0, yh2j = 0.0;
 #pragma unroll
-      for (int i = 0; i < S; ++i) y[i] = c.yh[0][i];
-      return OUT_MXSTEP;
+    for (int q = 0; q < S; ++q) {
+      const bool hit = (q == j);
+      ywj = hit ? opaque(c.ywt[q]) : ywj;
+      yj = hit ? opaque(c.y[q]) : yj;
+      fj = hit ? opaque(c.fac[q]) : fj;
+      s2j = hit ? opaque(c.save2[q]) : s2j;
+      yh2j = hit ? opaque(c.yh[2][q]) : yh2j;
     }
-    const int js = stp(c, f, k, mc);
-    if (js != 1) {
+    double f1[S];
+    if (mine) {
+      const double ys = dmax(fabs(ywj), fabs(yj));
+      double dy = fj * ys;
+      while (dy == 0.0) {
+        if (fj < 0.5) {
+          fj = dmin(100.0 * fj, 0.5);
+          dy = fj * ys;
+        } else {
+          dy = ys;
+          break;
+        }
+      }
+      dy = (c.nq == 1) ? copysign(dy, s2j) : copysign(dy, yh2j);
+      dy = (yj + dy) - yj;
+      const double yp = yj + dy;
 #pragma unroll
-      for (int i = 0; i < S; ++i) y[i] = c.y[i];
-      return (js == 2) ? OUT_STEP_ZERO : (js == 3) ? OUT_REPEATED_FAIL : OUT_SINGULAR;
+      for (int q = 0; q < S; ++q) c.y[q] = (q == j) ? yp : c.y[q];
+      rhs(kl, Mc, c.y, f1);
+#pragma unroll
+      for (int q = 0; q < S; ++q) c.y[q] = (q == j) ? yj : c.y[q];
+      const double factor = ((-l0) * c.h) / dy;
+      double col[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) col[i] = (f1[i] - c.save2[i]) * factor;
+      double diff = fabs(c.save2[0] - f1[0]);
+      double s2m = c.save2[0], s1m = f1[0];
+#pragma unroll
+      for (int i = 1; i < S; ++i) {
+        const double di = fabs(c.save2[i] - f1[i]);
+        if (di > diff) {
+          diff = di;
+          s2m = c.save2[i];
+          s1m = f1[i];
+        }
+      }
+      if (dmin(fabs(s2m), fabs(s1m)) > 0.0) {
+        const double scale = dmax(fabs(s2m), fabs(s1m));
+        if (diff > 0.5 * scale)
+          fj = dmax(k.facmin, fj * 0.5);
+        else if (k.br * scale <= diff && diff <= k.bl * scale)
+          fj = dmin(fj * 2.0, 0.5);
+        else if (diff < k.br * scale)
+          fj = dmin(k.bp * fj, 0.5);
+      }
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        const bool hit = (q == j);
+        c.fac[q] = hit ? fj : c.fac[q];
+#pragma unroll
+        for (int i = 0; i < S; ++i) c.P[q][i] = hit ? col[i] : c.P[q][i];
+      }
     }
-    c.jtask = 1;
-    if (c.t * hsign < tout * hsign) continue;
-    ntp(c, tout, y);
-    return OUT_OK;
+  }
+  if (mine) {
+    c.nfe += S;
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.P[i][i] = c.P[i][i] + 1.0;
+  }
+  // DGEFA for the lanes of the batch (the pivot ballots inside see only them)
+  if (mine) {
+    if (gefa(c.P, c.ipvt) != 0) c.ier = true;
   }
 }
 

@@ -69,6 +69,8 @@ struct StepArgs {
   int32_t *status;
   int32_t *nstep, *nfe, *nje;          // optional per-cell work counters
   unsigned long long *acc;              // [16] work/status accumulators
+  unsigned long long *queue;            // next unclaimed inner cell (persistent form)
+  int jac_pct;                          // run a Jacobian batch when >= this % of pending lanes want one
 };
 
 // Block-wide staging of the mechanism coefficients into LDS.
@@ -99,83 +101,164 @@ NRG_DI int wave_max_i32(int v)
 enum { ACC_INTEG = 0, ACC_NSTEP, ACC_NFE, ACC_NJE, ACC_NCORR, ACC_NATT, ACC_NSTEP_MAX, ACC_NFE_MAX,
        ACC_STATUS0 };   // ACC_STATUS0 + NRG_CELL_* (8 slots)
 
-// One chemistry step, one inner cell per lane (solve_chemical_kinetics, :224-346).
-__global__ void __launch_bounds__(256) chem_step_kernel(StepArgs a)
+// Zero the work accumulators and the queue head (a kernel, so the launch
+// sequence can be captured in a graph without memset nodes).
+__global__ void reset_kernel(unsigned long long *acc, int n)
+{
+  if ((int)threadIdx.x < n) acc[threadIdx.x] = 0ull;
+}
+
+// Load one cell and set it up (chemical_kinetics_solver.f90:228-265).
+// Returns false for a gated cell, whose outputs are written here (:341-346).
+NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long q, long long &off, Cell &c,
+                       double *kl, double (&Mc)[NM], unsigned long long (&cnt)[8])
+{
+  off = cell_offset(a.g, q);
+  const double T = a.T[off];
+  const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
+  if (!(T >= a.k.t_gate && bcv == 0)) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
+    a.Efp[off] = 0.0;
+    if (a.status) a.status[off] = NRG_CELL_GATED;
+    if (a.nstep) {
+      a.nstep[q] = 0;
+      a.nfe[q] = 0;
+      a.nje[q] = 0;
+    }
+    cnt[NRG_CELL_GATED]++;
+    return false;
+  }
+  const double rho = a.rho[off];
+#pragma unroll
+  for (int s = 0; s < S; ++s) c.y[s] = (a.Y[s][off] * rho) / mc.W[s];
+  rate_constants(mc, T, c.y, kl, Mc);
+  bdf_init(c, a.dt, a.k);
+  return true;
+}
+
+// Output assembly (:318-340) once the cell's DDRIV3 has returned.
+NRG_DI void finish_cell(const StepArgs &a, const MechCoef &mc, long long q, long long off,
+                        const Cell &c, unsigned long long (&cnt)[8], unsigned long long (&work)[6],
+                        int &mx_step, int &mx_fe)
+{
+  const double rho = a.rho[off];
+  double ef = 0.0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const double cin = (a.Y[s][off] * rho) / mc.W[s];
+    const double dc = c.y[s] - cin;
+    a.Yp[s][off] = (dc * mc.W[s]) / a.dt;
+    a.Yp2[s][off] = dc;
+    ef = ef - (mc.h_ref[s] * dc) / a.dt;
+  }
+  a.Efp[off] = ef;
+  const int status = (c.outcome == OUT_OK) ? NRG_CELL_OK
+                   : (c.outcome == OUT_MXSTEP || c.outcome == OUT_EPS) ? NRG_CELL_MXSTEP
+                   : (c.outcome == OUT_STEP_ZERO) ? NRG_CELL_STEP_ZERO
+                   : (c.outcome == OUT_REPEATED_FAIL) ? NRG_CELL_REPEATED_FAIL
+                   : NRG_CELL_SINGULAR;
+  if (a.status) a.status[off] = status;
+  if (a.nstep) {
+    a.nstep[q] = c.nstep;
+    a.nfe[q] = c.nfe;
+    a.nje[q] = c.nje;
+  }
+#pragma unroll
+  for (int st = 0; st < 8; ++st) cnt[st] += (status == st) ? 1ull : 0ull;
+  work[0] += 1;
+  work[1] += c.nstep;
+  work[2] += c.nfe;
+  work[3] += c.nje;
+  work[4] += c.ncorr;
+  work[5] += c.natt;
+  mx_step = max(mx_step, c.nstep);
+  mx_fe = max(mx_fe, c.nfe);
+}
+
+// One chemistry step (solve_chemical_kinetics, :224-346), one cell per lane.
+// PERSISTENT = false: lane i of block b owns inner cell 64*b + i (the plain
+//   launch).  PERSISTENT = true: a resident grid; a lane that finishes takes
+//   the next unclaimed cell from a.queue (one atomic per wave per refill).
+// Each iteration either evaluates f for the lanes that asked, or -- when at
+// least jac_pct % of the pending lanes wait for a Jacobian, or nothing else is
+// pending -- runs one batched FD Jacobian + LU for the waiting lanes.
+template <bool PERSISTENT>
+__global__ void __launch_bounds__(WAVE) chem_bdf_kernel(StepArgs a)
 {
   __shared__ MechCoef mc;
+  __shared__ double krate[2 * R * WAVE];
   stage_mech(mc, a.mech);
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = q < a.g.ninner;
-  int status = NRG_CELL_GATED;
-  int nstep = 0, nfe = 0, nje = 0, ncorr = 0, natt = 0;
-  if (live) {
-    const long long off = cell_offset(a.g, q);
-    const double T = a.T[off];
-    const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
-    if (T >= a.k.t_gate && bcv == 0) {
-      const double rho = a.rho[off];
-      double cin[S], y[S];
+  const int lane = threadIdx.x;
+  double *kl = krate + lane;
+  Cell c;
+  double Mc[NM];
 #pragma unroll
-      for (int s = 0; s < S; ++s) {
-        cin[s] = (a.Y[s][off] * rho) / mc.W[s];
-        y[s] = cin[s];
+  for (int i = 0; i < NM; ++i) Mc[i] = 0.0;
+  c.req = REQ_DONE;
+  long long q = -1, off = 0;
+  bool have = false, exhausted = false;
+  unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long work[6] = {0, 0, 0, 0, 0, 0};
+  int mx_step = 0, mx_fe = 0;
+  if (!PERSISTENT) {
+    q = (long long)blockIdx.x * WAVE + lane;
+    if (q < a.g.ninner) have = start_cell(a, mc, q, off, c, kl, Mc, cnt);
+  }
+  for (;;) {
+    if (PERSISTENT) {
+      bool need = !have && !exhausted;
+      while (wave_any(need)) {
+        const unsigned long long mask = __ballot(need ? 1 : 0);
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader, WAVE);
+        if (need) {
+          q = (long long)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+          if (q >= a.g.ninner)
+            exhausted = true;
+          else
+            have = start_cell(a, mc, q, off, c, kl, Mc, cnt);
+        }
+        need = !have && !exhausted;
       }
-      Rates kr;
-      rate_constants(mc, T, cin, kr);
-      Cell c;
-      auto f = [&kr](const double (&z)[S], double (&zd)[S]) { rhs(kr, z, zd); };
-      const int out = ddriv3(c, f, y, a.dt, a.k, mc);
-      double ef = 0.0;
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        const double dc = y[s] - cin[s];
-        a.Yp[s][off] = (dc * mc.W[s]) / a.dt;
-        a.Yp2[s][off] = dc;
-      }
-#pragma unroll
-      for (int s = 0; s < S; ++s) ef = ef - (mc.h_ref[s] * (y[s] - cin[s])) / a.dt;
-      a.Efp[off] = ef;
-      status = (out == OUT_OK) ? NRG_CELL_OK
-             : (out == OUT_MXSTEP) ? NRG_CELL_MXSTEP
-             : (out == OUT_STEP_ZERO) ? NRG_CELL_STEP_ZERO
-             : (out == OUT_REPEATED_FAIL) ? NRG_CELL_REPEATED_FAIL
-             : NRG_CELL_SINGULAR;
-      nstep = c.nstep;
-      nfe = c.nfe;
-      nje = c.nje;
-      ncorr = c.ncorr;
-      natt = c.natt;
-    } else {
-#pragma unroll
-      for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
-      a.Efp[off] = 0.0;
     }
-    if (a.status) a.status[off] = status;
-    if (a.nstep) {
-      a.nstep[q] = nstep;
-      a.nfe[q] = nfe;
-      a.nje[q] = nje;
+    if (!wave_any(have)) break;
+    const bool wantj = have && c.req == REQ_JAC;
+    const bool wantf = have && c.req == REQ_F;
+    const int nj = __popcll(__ballot(wantj ? 1 : 0));
+    const int nf = __popcll(__ballot(wantf ? 1 : 0));
+    const bool run_jac = nj > 0 && (nf == 0 || nj * 100 >= a.jac_pct * (nj + nf));
+    if (run_jac)
+      jac_batch(c, wantj, a.k, mc, kl, Mc);
+    else if (wantf)
+      rhs(kl, Mc, c.y, c.save2);
+    if (have && (c.req == REQ_NONE || (run_jac ? wantj : wantf))) {
+      c.req = REQ_NONE;
+      advance(c, a.dt, a.k, mc);
     }
+    if (have && c.req == REQ_DONE) {
+      finish_cell(a, mc, q, off, c, cnt, work, mx_step, mx_fe);
+      have = false;
+    }
+    if (!PERSISTENT && !wave_any(have)) break;
   }
   // work counters: one atomic per wave and slot
-  const bool integ = live && status != NRG_CELL_GATED;
-  const unsigned long long vals[6] = {integ ? 1ull : 0ull, (unsigned long long)nstep,
-                                      (unsigned long long)nfe, (unsigned long long)nje,
-                                      (unsigned long long)ncorr, (unsigned long long)natt};
-  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    const unsigned long long w = wave_sum_u64(vals[i]);
+    const unsigned long long w = wave_sum_u64(work[i]);
     if (lane == 0 && w) atomicAdd(&a.acc[ACC_INTEG + i], w);
   }
-  const int mx_step = wave_max_i32(nstep), mx_fe = wave_max_i32(nfe);
+  mx_step = wave_max_i32(mx_step);
+  mx_fe = wave_max_i32(mx_fe);
   if (lane == 0) {
     atomicMax(&a.acc[ACC_NSTEP_MAX], (unsigned long long)mx_step);
     atomicMax(&a.acc[ACC_NFE_MAX], (unsigned long long)mx_fe);
   }
 #pragma unroll
   for (int st = 0; st < 6; ++st) {
-    const unsigned long long w = wave_sum_u64((live && status == st) ? 1ull : 0ull);
+    const unsigned long long w = wave_sum_u64(cnt[st]);
     if (lane == 0 && w) atomicAdd(&a.acc[ACC_STATUS0 + st], w);
   }
 }
@@ -322,9 +405,11 @@ struct nrg_chem_handle {
   MechCoef *dev_mech = nullptr;
   BdfConst k{};
   nrg_chem_opts opts{};
-  unsigned long long *dev_acc = nullptr;
+  unsigned long long *dev_acc = nullptr;   // [16] accumulators + [1] queue head
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int block = 256;
+  int form = NRG_FORM_PLAIN;
+  int jac_pct = 50;
+  int persistent_blocks = 0;               // resident one-wave blocks of the persistent form
   // staging buffers for the host-pointer entry point
   std::vector<void *> stage;
   size_t stage_cells = 0;
@@ -502,7 +587,9 @@ void nrg_chem_default_opts(nrg_chem_opts *o)
   o->mxord = 5;                       // :282
   o->jacobian = NRG_JAC_FD_SLATEC;
   o->device = 0;
-  o->block_size = 0;
+  o->form = NRG_FORM_PLAIN;
+  o->jac_batch_pct = 0;
+  o->waves_per_cu = 0;
 }
 
 const char *nrg_chem_strerror(int st)
@@ -534,8 +621,12 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     return set_error(NRG_CHEM_E_ARG, "invalid integrator options");
   if (o.jacobian != NRG_JAC_FD_SLATEC)
     return set_error(NRG_CHEM_E_ARG, "jacobian mode %d not available", o.jacobian);
-  if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64))
-    return set_error(NRG_CHEM_E_ARG, "block_size must be 64, 128, 192 or 256");
+  if (o.form != NRG_FORM_PLAIN && o.form != NRG_FORM_PERSISTENT)
+    return set_error(NRG_CHEM_E_ARG, "unknown form %d", o.form);
+  if (o.jac_batch_pct < 0 || o.jac_batch_pct > 100)
+    return set_error(NRG_CHEM_E_ARG, "jac_batch_pct must be in 0..100");
+  if (o.waves_per_cu < 0 || o.waves_per_cu > 32)
+    return set_error(NRG_CHEM_E_ARG, "waves_per_cu must be in 0..32");
   auto *h = new (std::nothrow) nrg_chem_handle();
   if (!h) return set_error(NRG_CHEM_E_ARG, "out of host memory");
   int st = build_mech(mech, h->host_mech);
@@ -547,7 +638,8 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
   h->opts = o;
   h->device = o.device;
   h->k = make_const(o);
-  h->block = o.block_size ? o.block_size : 256;
+  h->form = o.form;
+  h->jac_pct = o.jac_batch_pct ? o.jac_batch_pct : 50;
   auto fail = [&](int s) {
     nrg_chem_destroy(h);
     return s;
@@ -556,8 +648,19 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
   if (hipStreamCreate(&h->stream) != hipSuccess) return fail(set_error(NRG_CHEM_E_HIP, "hipStreamCreate"));
   if (hipMalloc(&h->dev_mech, sizeof(MechCoef)) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipMalloc mech"));
-  if (hipMalloc(&h->dev_acc, 16 * sizeof(unsigned long long)) != hipSuccess)
+  if (hipMalloc(&h->dev_acc, 17 * sizeof(unsigned long long)) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipMalloc acc"));
+  {
+    hipDeviceProp_t prop;
+    int per_cu = 0;
+    if (hipGetDeviceProperties(&prop, h->device) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, chem_bdf_kernel<true>, WAVE, 0) !=
+            hipSuccess)
+      return fail(set_error(NRG_CHEM_E_HIP, "occupancy query"));
+    if (o.waves_per_cu > 0) per_cu = o.waves_per_cu;
+    if (per_cu < 1) per_cu = 1;
+    h->persistent_blocks = per_cu * prop.multiProcessorCount;
+  }
   if (hipMemcpy(h->dev_mech, &h->host_mech, sizeof(MechCoef), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipMemcpy mech"));
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
@@ -614,14 +717,23 @@ static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const d
   a.nfe = nfe;
   a.nje = nje;
   a.acc = h->dev_acc;
+  a.queue = h->dev_acc + 16;
+  a.jac_pct = h->jac_pct;
   HIP_TRY(hipSetDevice(h->device));
-  HIP_TRY(hipMemsetAsync(h->dev_acc, 0, 16 * sizeof(unsigned long long), h->stream));
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, h->stream, h->dev_acc, 17);
+  HIP_TRY(hipGetLastError());
   unsigned long long acc[16] = {};
   float ms = 0.f;
   if (a.g.ninner > 0) {
-    const long long nb = (a.g.ninner + h->block - 1) / h->block;
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    hipLaunchKernelGGL(chem_step_kernel, dim3((unsigned)nb), dim3(h->block), 0, h->stream, a);
+    if (h->form == NRG_FORM_PERSISTENT) {
+      const long long need = (a.g.ninner + WAVE - 1) / WAVE;
+      const long long nb = std::min<long long>(h->persistent_blocks, need);
+      hipLaunchKernelGGL(chem_bdf_kernel<true>, dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
+    } else {
+      const long long nb = (a.g.ninner + WAVE - 1) / WAVE;
+      hipLaunchKernelGGL(chem_bdf_kernel<false>, dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipMemcpyAsync(acc, h->dev_acc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
