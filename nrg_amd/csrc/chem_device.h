@@ -69,6 +69,14 @@ struct BdfConst {
 NRG_DI double dmax(double a, double b) { return (a >= b) ? a : b; }
 NRG_DI double dmin(double a, double b) { return (a <= b) ? a : b; }
 
+// One out-of-line copy of each transcendental: the rate constants call them
+// ~120 times per cell and the step-size choice a few times per step; inlined,
+// every call site would carry its own 100+-instruction expansion.
+__device__ __noinline__ double nl_pow(double x, double y) { return pow(x, y); }
+__device__ __noinline__ double nl_exp(double x) { return exp(x); }
+__device__ __noinline__ double nl_log(double x) { return log(x); }
+__device__ __noinline__ double nl_log10(double x) { return log10(x); }
+
 // A value pinned in a VGPR.  Select chains over register arrays go through it,
 // so the optimiser cannot turn "select between loads" into one load through a
 // selected address (which would keep the whole array in private memory).
@@ -149,7 +157,7 @@ NRG_DI double s_molar(const MechCoef &mc, double T, int s)
   sm = sm * temp + d[1];
   sm = sm * temp + d[2];
   sm = sm * temp + a[1];
-  return (sm * temp + a[6]) + a[0] * log(temp);
+  return (sm * temp + a[6]) + a[0] * nl_log(temp);
 }
 
 NRG_DI double cp_molar(const MechCoef &mc, double T, int s)
@@ -213,39 +221,39 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
     const int rt = topo::rtype[r];
     double kf = 0.0, kr = 0.0;
     if (rt == 0 || rt == 1 || rt == 5 || rt == 6) {
-      kf = (mc.A[r] * pow(T, mc.beta[r])) * exp(mc.mE_R[r] / T);
+      kf = (mc.A[r] * nl_pow(T, mc.beta[r])) * nl_exp(mc.mE_R[r] / T);
     } else if (rt == 2 || rt == 7) {
-      const double k0 = (mc.A_low[r] * pow(T, mc.beta_low[r])) * exp(mc.mElow_R[r] / T);
-      const double kinf = (mc.A[r] * pow(T, mc.beta[r])) * exp(mc.mE_R[r] / T);
+      const double k0 = (mc.A_low[r] * nl_pow(T, mc.beta_low[r])) * nl_exp(mc.mElow_R[r] / T);
+      const double kinf = (mc.A[r] * nl_pow(T, mc.beta[r])) * nl_exp(mc.mE_R[r] / T);
       const double pr = (k0 * m) / kinf;
       kf = kinf * (pr / (1.0 + pr));
     } else if (rt == 3) {
-      double k0 = exp(mc.mElow_R[r] / T);
-      double kinf = exp(mc.mE_R[r] / T);
-      k0 = (mc.A_low[r] * pow(T, mc.beta_low[r])) * k0;
-      kinf = (mc.A[r] * pow(T, mc.beta[r])) * kinf;
+      double k0 = nl_exp(mc.mElow_R[r] / T);
+      double kinf = nl_exp(mc.mE_R[r] / T);
+      k0 = (mc.A_low[r] * nl_pow(T, mc.beta_low[r])) * k0;
+      kinf = (mc.A[r] * nl_pow(T, mc.beta[r])) * kinf;
       const double pr = (k0 * m) / kinf;
-      const double fcent = (mc.one_m_alpha[r] * exp(-(T / mc.troe[1][r])) +
-                            mc.troe[0][r] * exp(-(T / mc.troe[2][r]))) +
-                           exp(-(mc.troe[3][r] / T));
-      const double lf = log10(fcent);
+      const double fcent = (mc.one_m_alpha[r] * nl_exp(-(T / mc.troe[1][r])) +
+                            mc.troe[0][r] * nl_exp(-(T / mc.troe[2][r]))) +
+                           nl_exp(-(mc.troe[3][r] / T));
+      const double lf = nl_log10(fcent);
       const double c = -0.4 - 0.67 * lf;
       const double n = 0.75 - 1.27 * lf;
       const double d = 0.14;
       double blend;
       if (pr != 0.0) {
-        const double lp = log10(pr);
+        const double lp = nl_log10(pr);
         const double x = (lp + c) / (n - d * (lp + c));
-        blend = pow(fcent, 1.0 / (1.0 + x * x));
+        blend = nl_pow(fcent, 1.0 / (1.0 + x * x));
       } else {
-        blend = pow(fcent, -1.0 / d);
+        blend = nl_pow(fcent, -1.0 / d);
       }
       kf = (kinf * (pr / (1.0 + pr))) * blend;
     }
     if (rt == 0 || rt == 1 || rt == 2 || rt == 3) {
-      const double Kp = exp(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
-      // K_c = K_p * ((P_atm/R_u)/T)**d_nu; d_nu = 0 gives pow(x, 0) = 1 exactly
-      const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * pow(mc.patm_r / T, d_nu);
+      const double Kp = nl_exp(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
+      // K_c = K_p * ((P_atm/R_u)/T)**d_nu; d_nu = 0 gives nl_pow(x, 0) = 1 exactly
+      const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * nl_pow(mc.patm_r / T, d_nu);
       kr = kf / Kc;
     }
     kl[r * WAVE] = kf;
@@ -292,9 +300,14 @@ NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[S], 
 // DNRM2 (slatec/dnrm2.f90:84-160): Lawson's phases as a per-element state machine.
 // st: 0 = start (label 30), 1 = phase 1 (50), 2 = phase 2 (70), 3 = phase 4 (110),
 //     4 = phase 3 (the DO 95 loop, unscaled sum).
+// One out-of-line copy serves the seven call sites; the vector arrives in
+// argument VGPRs.
 // ---------------------------------------------------------------------------
-NRG_DI double dnrm2(const double (&x)[S])
+static_assert(S == 9, "dnrm2_vals takes the 9 species of the compiled mechanism");
+__device__ __noinline__ double dnrm2_vals(double x0, double x1, double x2, double x3, double x4,
+                                          double x5, double x6, double x7, double x8)
 {
+  const double x[S] = {x0, x1, x2, x3, x4, x5, x6, x7, x8};
   const double cutlo = 8.232e-11;
   const double hitest = 1.304e19 / (double)S;
   double sum = 0.0, xmax = 0.0;
@@ -364,6 +377,11 @@ NRG_DI double dnrm2(const double (&x)[S])
     }
   }
   return (st == 4) ? sqrt(sum) : xmax * sqrt(sum);
+}
+
+NRG_DI double dnrm2(const double (&x)[S])
+{
+  return dnrm2_vals(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8]);
 }
 
 // ---------------------------------------------------------------------------
@@ -461,6 +479,8 @@ enum Pc : int {
   PC_CORR_F,       // after f(y) of a corrector iteration
   PC_FAIL_CONV,    // label 160: corrector failure
   PC_ERRTEST,      // label 170: error test, accept, step/order selection
+  PC_RETRACT,      // retract the prediction after a failed attempt
+  PC_SCALE,        // apply a pending DDSCL rescale, then continue
   PC_DONE
 };
 enum Req : int { REQ_NONE = 0, REQ_F = 1, REQ_JAC = 2, REQ_DONE = 3 };
@@ -476,6 +496,9 @@ struct Cell {
   int nq, nwait, jtask, jstepl, nstep, nfe, nje, ncorr, natt;
   int ntry, nfail, iter, pc, req, outcome;
   bool convrg, ier, evaljc, ntl_restart;
+  // pending retraction / rescale (stages R and S)
+  double rh, etest;
+  bool fail_err, do_scale, rmax_after, after_accept;
 };
 
 NRG_DI double el_of(const MechCoef &mc, int j, int nq) { return mc.el[nq - 1][j - 1]; }
@@ -628,281 +651,308 @@ NRG_DI void bdf_init(Cell &c, double tout, const BdfConst &k)
   c.outcome = OUT_OK;
 }
 
+// Stage guards: a lane runs a stage when it sits at that program counter and
+// has not yet asked for anything in this pass; a stage's code is skipped by
+// the whole wave when no lane needs it.
+NRG_DI bool at(const Cell &c, int pc) { return c.req == REQ_NONE && c.pc == pc; }
+
 // Run the cell's program until it needs f(y) (req = REQ_F, y set), a
 // Jacobian (REQ_JAC) or is finished (REQ_DONE).  On REQ_F resumption the
 // caller has put f(y) into save2.  hsign is +1 throughout (tout > 0).
+//
+// DDRIV3/DDSTP's control flow is laid out as seven stages in a fixed order,
+// so that every transition taken in one pass goes forward:
+//   1 DDNTL tail   2 after f / Jacobian   3 corrector (label 130)
+//   4 error test, accept, step/order choice (label 170)
+//   5 corrector failure (label 160)   6 DDRIV3 label 260   7 attempt (label 100)
+// Each stage's code exists once; a lane may pass several stages per call.
 NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
 {
-  for (;;) {
-    switch (c.pc) {
-      case PC_260: {
+  // ---- 1: rest of DDNTL after its f(y) (slatec/ddntl.f90:68,152-161)
+  if (wave_any(at(c, PC_NTL_F)) && at(c, PC_NTL_F)) {
+    c.nfe++;
 #pragma unroll
-        for (int i = 0; i < S; ++i) c.ywt[i] = dmax(k.ewt, fabs(c.y[i]));
+    for (int i = 0; i < S; ++i) c.save1[i] = c.save2[i] / dmax(1.0, c.ywt[i]);
+    const double sum = dnrm2(c.save1) / k.sqrtn;
+    if (sum > k.eps / fabs(c.h)) c.h = copysign(k.eps / sum, c.h);
 #pragma unroll
-        for (int i = 0; i < S; ++i) c.save2[i] = c.y[i] / c.ywt[i];
-        double sum = dnrm2(c.save2) / k.sqrtn;
-        sum = dmax(1.0, sum);
-        if (k.eps < sum * k.uround) {
-          copy_yh0_to_y(c);
-          finish(c, OUT_EPS);
-          return;
-        }
-        if (c.nstep == k.mxstep) {
-          copy_yh0_to_y(c);
-          finish(c, OUT_MXSTEP);
-          return;
-        }
-        c.told = c.t;                                    // DDSTP entry
-        c.ntry = 0;
-        c.nfail = 0;
-        if (c.jtask <= 0) {
-          ntl_head(c);
-          c.ntl_restart = false;
-          request_f(c, PC_NTL_F);
-          return;
-        }
-        c.pc = PC_ATTEMPT;
-        break;
-      }
-      case PC_NTL_F: {                                   // rest of DDNTL (:68,152-161)
-        c.nfe++;
+    for (int i = 0; i < S; ++i) c.yh[1][i] = c.h * c.save2[i];
 #pragma unroll
-        for (int i = 0; i < S; ++i) c.save1[i] = c.save2[i] / dmax(1.0, c.ywt[i]);
-        const double sum = dnrm2(c.save1) / k.sqrtn;
-        if (sum > k.eps / fabs(c.h)) c.h = copysign(k.eps / sum, c.h);
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.yh[1][i] = c.h * c.save2[i];
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.fac[i] = k.sqrt_uround;
-        if (c.ntl_restart) c.rmax = 10.0;
-        if (c.h == 0.0) {
-          copy_yh0_to_y(c);
-          finish(c, OUT_STEP_ZERO);
-          return;
-        }
-        c.pc = PC_ATTEMPT;
-        break;
-      }
-      case PC_ATTEMPT: {                                 // label 100
-        c.ntry++;
-        if (c.ntry > 50) {
-          finish(c, OUT_REPEATED_FAIL);
-          return;
-        }
-        c.natt++;
-        c.t = c.t + c.h;
-        psc<true>(c.nq, c.yh);
-        c.evaljc = (fabs(c.rc - 1.0) > 0.3) || (c.nstep >= c.jstepl + 10);
-        c.iter = 0;                                      // label 110
-        copy_yh0_to_y(c);
-        request_f(c, PC_PRED_F);
-        return;
-      }
-      case PC_PRED_F: {
-        c.nfe++;
-        if (c.evaljc || c.ier) {
-          c.pc = PC_JAC_DONE;
-          c.req = REQ_JAC;
-          return;
-        }
+    for (int i = 0; i < S; ++i) c.fac[i] = k.sqrt_uround;
+    if (c.ntl_restart) c.rmax = 10.0;
+    if (c.h == 0.0) {
+      copy_yh0_to_y(c);
+      finish(c, OUT_STEP_ZERO);
+    } else {
+      c.pc = PC_ATTEMPT;
+    }
+  }
+  // ---- 2: after f(y_pred), after a corrector f, after the Jacobian
+  if (wave_any(c.req == REQ_NONE && (c.pc == PC_PRED_F || c.pc == PC_JAC_DONE || c.pc == PC_CORR_F))) {
+    if (at(c, PC_PRED_F)) {
+      c.nfe++;
+      if (c.evaljc || c.ier) {
+        c.pc = PC_JAC_DONE;
+        c.req = REQ_JAC;
+      } else {
 #pragma unroll
         for (int i = 0; i < S; ++i) c.save1[i] = 0.0;
         c.pc = PC_CORR;
-        break;
       }
-      case PC_JAC_DONE: {
-        if (c.ier) {
-          c.pc = PC_FAIL_CONV;
-          break;
-        }
+    } else if (at(c, PC_JAC_DONE)) {
+      if (c.ier) {
+        c.pc = PC_FAIL_CONV;
+      } else {
         c.convrg = false;
         c.rc = 1.0;
         c.jstepl = c.nstep;
 #pragma unroll
         for (int i = 0; i < S; ++i) c.save1[i] = 0.0;
         c.pc = PC_CORR;
-        break;
       }
-      case PC_CORR_F: {
-        c.nfe++;
-        c.pc = PC_CORR;
-        break;
-      }
-      case PC_CORR: {                                    // label 130
-        const double d = cor(c, k);
-        if (c.iter > 0) c.trend = dmax(0.9 * c.trend, d / c.d1);
-        c.d1 = d;
-        c.ctest = dmin(2.0 * c.trend, 1.0) * d;
-        if (c.ctest <= k.eps) {
-          c.pc = PC_ERRTEST;
-          break;
-        }
-        c.iter++;
-        if (c.iter < 3) {
-          const double l0 = el_of(mc, 1, c.nq);
+    } else if (at(c, PC_CORR_F)) {
+      c.nfe++;
+      c.pc = PC_CORR;
+    }
+  }
+  // ---- 3: corrector iteration, label 130 (slatec/ddstp.f90:152-205)
+  if (wave_any(at(c, PC_CORR)) && at(c, PC_CORR)) {
+    const double d = cor(c, k);
+    if (c.iter > 0) c.trend = dmax(0.9 * c.trend, d / c.d1);
+    c.d1 = d;
+    c.ctest = dmin(2.0 * c.trend, 1.0) * d;
+    if (c.ctest <= k.eps) {
+      c.pc = PC_ERRTEST;
+    } else {
+      c.iter++;
+      if (c.iter < 3) {
+        const double l0 = el_of(mc, 1, c.nq);
 #pragma unroll
-          for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i] + l0 * c.save1[i];
-          request_f(c, PC_CORR_F);
-          return;
-        }
-        if (c.convrg) {                                  // stale Jacobian: refresh (label 110)
-          c.evaljc = true;
-          c.iter = 0;
-          copy_yh0_to_y(c);
-          request_f(c, PC_PRED_F);
-          return;
-        }
+        for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i] + l0 * c.save1[i];
+        request_f(c, PC_CORR_F);
+      } else if (c.convrg) {                           // stale Jacobian: refresh (label 110)
+        c.evaljc = true;
+        c.iter = 0;
+        copy_yh0_to_y(c);
+        request_f(c, PC_PRED_F);
+      } else {
         c.pc = PC_FAIL_CONV;
-        break;
       }
-      case PC_FAIL_CONV: {                               // label 160
-        c.t = c.told;
-        psc<false>(c.nq, c.yh);
+    }
+  }
+  // ---- 4: error test, accept, step/order selection, label 170 (:222-432)
+  if (wave_any(at(c, PC_ERRTEST)) && at(c, PC_ERRTEST)) {
+    c.convrg = true;
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.save2[i] = c.save1[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+    const double etest = dnrm2(c.save2) / (tq_of(mc, 2, c.nq) * k.sqrtn);
+    c.etest = etest;
+    if (etest > k.eps) {
+      c.fail_err = true;                               // -> stage R (:238-289)
+      c.pc = PC_RETRACT;
+    } else {
+      // accepted step (slatec/ddstp.f90:291-300)
+      c.nstep++;
+#pragma unroll
+      for (int j = 1; j <= MAXORD + 1; ++j) {
+        if (j <= c.nq + 1) {
+          const double e = el_of(mc, j, c.nq);
+#pragma unroll
+          for (int i = 0; i < S; ++i) c.yh[j - 1][i] = c.yh[j - 1][i] + e * c.save1[i];
+        }
+      }
+      copy_yh0_to_y(c);
+      c.do_scale = false;
+      c.rmax_after = false;
+      c.after_accept = true;
+      // step-size and order selection (slatec/ddstp.f90:360-432)
+      if (c.jtask == 0 || c.jtask == 2) {
+        const double rh = 1.0 / dmax(k.uround, 1.2 * nl_pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
+        c.rh = rh;
+        c.do_scale = rh > 1.0;
+      } else if (c.nwait > 1) {
+        c.nwait = c.nwait - 1;
+        if (c.nwait == 1 && c.nq < k.maxord) {
+#pragma unroll
+          for (int i = 0; i < S; ++i) c.yh[MAXORD][i] = c.save1[i];
+        }
+      } else {
+        double rh1 = 0.0, rh3 = 0.0;
+        if (c.nq != 1) {
+          double col[S];
+          yh_col(c, c.nq, col);
+#pragma unroll
+          for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+          const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
+          rh1 = 1.0 / dmax(k.uround, 1.3 * nl_pow(erdn / k.eps, 1.0 / (double)c.nq));
+        }
+        const double rh2 = 1.0 / dmax(k.uround, 1.2 * nl_pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
+        if (c.nq != k.maxord) {
+#pragma unroll
+          for (int i = 0; i < S; ++i)
+            c.save2[i] = (c.save1[i] - c.yh[MAXORD][i]) / dmax(fabs(c.y[i]), c.ywt[i]);
+          const double erup = dnrm2(c.save2) / (tq_of(mc, 3, c.nq) * k.sqrtn);
+          rh3 = 1.0 / dmax(k.uround, 1.4 * nl_pow(erup / k.eps, 1.0 / (double)(c.nq + 2)));
+        }
+        double rh;
+        bool change = true;
+        if (rh1 > rh2 && rh1 >= rh3) {
+          rh = rh1;
+          if (rh <= 1.0) {
+            change = false;
+          } else {
+            c.nq = c.nq - 1;
+            c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
+          }
+        } else if (rh2 >= rh1 && rh2 >= rh3) {
+          rh = rh2;
+          if (rh <= 1.0) change = false;
+        } else {
+          rh = rh3;
+          if (rh <= 1.0) {
+            change = false;
+          } else {
+            const double e = el_of(mc, c.nq + 1, c.nq);
+            const double den = (double)(c.nq + 1);
+            const int dst = c.nq + 1;
+            double newc[S];
+#pragma unroll
+            for (int i = 0; i < S; ++i) newc[i] = (c.save1[i] * e) / den;
+#pragma unroll
+            for (int j = 2; j <= MAXORD; ++j) {
+#pragma unroll
+              for (int i = 0; i < S; ++i) c.yh[j][i] = (dst == j) ? newc[i] : c.yh[j][i];
+            }
+            c.nq = c.nq + 1;
+            c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq - 1);
+          }
+        }
+        c.rh = rh;
+        c.do_scale = change;
+        c.rmax_after = change;
         c.nwait = c.nq + 2;
+      }
+      c.pc = PC_SCALE;
+    }
+  }
+  // ---- 5: corrector failure, label 160 (slatec/ddstp.f90:206-217)
+  if (wave_any(at(c, PC_FAIL_CONV)) && at(c, PC_FAIL_CONV)) {
+    c.fail_err = false;
+    c.pc = PC_RETRACT;
+  }
+  // ---- R: retract the prediction, then the failure's step-size choice
+  if (wave_any(at(c, PC_RETRACT)) && at(c, PC_RETRACT)) {
+    c.t = c.told;
+    psc<false>(c.nq, c.yh);
+    c.after_accept = false;
+    c.rmax_after = false;
+    if (!c.fail_err) {                                 // label 160
+      c.nwait = c.nq + 2;
+      if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
+      const double rh = (c.iter == 0) ? 0.3 : 0.9 * nl_pow(k.eps / c.ctest, 0.2);
+      if (rh * c.h == 0.0) {
+        copy_yh0_to_y(c);
+        finish(c, OUT_STEP_ZERO);
+      } else {
+        c.rh = rh;
+        c.do_scale = true;
+        c.pc = PC_SCALE;
+      }
+    } else {                                           // error test failed (:238-289)
+      c.nfail++;
+      if (c.nfail < 3 || c.nq == 1) {
         if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
-        const double rh = (c.iter == 0) ? 0.3 : 0.9 * pow(k.eps / c.ctest, 0.2);
+        const double rh2 = 1.0 / (1.2 * nl_pow(c.etest / k.eps, 1.0 / (double)(c.nq + 1)));
+        double rh = rh2;
+        if (c.nq > 1) {
+          double col[S];
+          yh_col(c, c.nq, col);
+#pragma unroll
+          for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+          const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
+          const double rh1 = 1.0 / dmax(1.0, 1.3 * nl_pow(erdn / k.eps, 1.0 / (double)c.nq));
+          if (rh2 < rh1) {
+            c.nq = c.nq - 1;
+            c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
+            rh = rh1;
+          }
+        }
+        c.nwait = c.nq + 2;
         if (rh * c.h == 0.0) {
           copy_yh0_to_y(c);
           finish(c, OUT_STEP_ZERO);
-          return;
-        }
-        scl(c, rh, k);
-        c.pc = PC_ATTEMPT;
-        break;
-      }
-      case PC_ERRTEST: {                                 // label 170
-        c.convrg = true;
-#pragma unroll
-        for (int i = 0; i < S; ++i) c.save2[i] = c.save1[i] / dmax(fabs(c.y[i]), c.ywt[i]);
-        const double etest = dnrm2(c.save2) / (tq_of(mc, 2, c.nq) * k.sqrtn);
-        if (etest > k.eps) {
-          c.t = c.told;
-          psc<false>(c.nq, c.yh);
-          c.nfail++;
-          if (c.nfail < 3 || c.nq == 1) {
-            if (c.jtask != 0 && c.jtask != 2) c.rmax = 2.0;
-            const double rh2 = 1.0 / (1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
-            double rh = rh2;
-            if (c.nq > 1) {
-              double col[S];
-              yh_col(c, c.nq, col);
-#pragma unroll
-              for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
-              const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
-              const double rh1 = 1.0 / dmax(1.0, 1.3 * pow(erdn / k.eps, 1.0 / (double)c.nq));
-              if (rh2 < rh1) {
-                c.nq = c.nq - 1;
-                c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
-                rh = rh1;
-              }
-            }
-            c.nwait = c.nq + 2;
-            if (rh * c.h == 0.0) {
-              copy_yh0_to_y(c);
-              finish(c, OUT_STEP_ZERO);
-              return;
-            }
-            scl(c, rh, k);
-            c.pc = PC_ATTEMPT;
-            break;
-          }
-          c.nfail = 0;                                   // restart at order 1 (:275-288)
-          c.jtask = 2;
-          copy_yh0_to_y(c);
-          ntl_head(c);
-          c.ntl_restart = true;
-          request_f(c, PC_NTL_F);
-          return;
-        }
-        // accepted step (slatec/ddstp.f90:291-300)
-        c.nstep++;
-#pragma unroll
-        for (int j = 1; j <= MAXORD + 1; ++j) {
-          if (j <= c.nq + 1) {
-            const double e = el_of(mc, j, c.nq);
-#pragma unroll
-            for (int i = 0; i < S; ++i) c.yh[j - 1][i] = c.yh[j - 1][i] + e * c.save1[i];
-          }
-        }
-        copy_yh0_to_y(c);
-        // step-size and order selection (slatec/ddstp.f90:360-432)
-        if (c.jtask == 0 || c.jtask == 2) {
-          const double rh = 1.0 / dmax(k.uround, 1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
-          if (rh > 1.0) scl(c, rh, k);
-        } else if (c.nwait > 1) {
-          c.nwait = c.nwait - 1;
-          if (c.nwait == 1 && c.nq < k.maxord) {
-#pragma unroll
-            for (int i = 0; i < S; ++i) c.yh[MAXORD][i] = c.save1[i];
-          }
         } else {
-          double rh1 = 0.0, rh3 = 0.0;
-          if (c.nq != 1) {
-            double col[S];
-            yh_col(c, c.nq, col);
-#pragma unroll
-            for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
-            const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
-            rh1 = 1.0 / dmax(k.uround, 1.3 * pow(erdn / k.eps, 1.0 / (double)c.nq));
-          }
-          const double rh2 = 1.0 / dmax(k.uround, 1.2 * pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
-          if (c.nq != k.maxord) {
-#pragma unroll
-            for (int i = 0; i < S; ++i)
-              c.save2[i] = (c.save1[i] - c.yh[MAXORD][i]) / dmax(fabs(c.y[i]), c.ywt[i]);
-            const double erup = dnrm2(c.save2) / (tq_of(mc, 3, c.nq) * k.sqrtn);
-            rh3 = 1.0 / dmax(k.uround, 1.4 * pow(erup / k.eps, 1.0 / (double)(c.nq + 2)));
-          }
-          double rh;
-          bool change = true;
-          if (rh1 > rh2 && rh1 >= rh3) {
-            rh = rh1;
-            if (rh <= 1.0) {
-              change = false;
-            } else {
-              c.nq = c.nq - 1;
-              c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq + 1);
-            }
-          } else if (rh2 >= rh1 && rh2 >= rh3) {
-            rh = rh2;
-            if (rh <= 1.0) change = false;
-          } else {
-            rh = rh3;
-            if (rh <= 1.0) {
-              change = false;
-            } else {
-              const double e = el_of(mc, c.nq + 1, c.nq);
-              const double den = (double)(c.nq + 1);
-              const int dst = c.nq + 1;
-#pragma unroll
-              for (int j = 2; j <= MAXORD; ++j) {
-#pragma unroll
-                for (int i = 0; i < S; ++i)
-                  c.yh[j][i] = (dst == j) ? (c.save1[i] * e) / den : c.yh[j][i];
-              }
-              c.nq = c.nq + 1;
-              c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq - 1);
-            }
-          }
-          if (change) {
-            scl(c, rh, k);
-            c.rmax = 10.0;
-          }
-          c.nwait = c.nq + 2;
+          c.rh = rh;
+          c.do_scale = true;
+          c.pc = PC_SCALE;
         }
-        // DDSTP returned JSTATE = 1; DDRIV3 label 470 (:1371, NTASK = 1 test :1443-1449)
-        c.jtask = 1;
-        if (c.t < tout) {
-          c.pc = PC_260;
-          break;
-        }
+      } else {
+        c.nfail = 0;                                   // restart at order 1 (:275-288)
+        c.jtask = 2;
+        copy_yh0_to_y(c);
+        ntl_head(c);
+        c.ntl_restart = true;
+        request_f(c, PC_NTL_F);
+      }
+    }
+  }
+  // ---- S: pending DDSCL, then the next attempt or DDRIV3's label 470
+  if (wave_any(at(c, PC_SCALE)) && at(c, PC_SCALE)) {
+    if (c.do_scale) scl(c, c.rh, k);
+    if (c.rmax_after) c.rmax = 10.0;
+    if (!c.after_accept) {
+      c.pc = PC_ATTEMPT;
+    } else {
+      // DDSTP returned JSTATE = 1; DDRIV3 label 470 (:1371, NTASK = 1 test :1443-1449)
+      c.jtask = 1;
+      if (c.t < tout) {
+        c.pc = PC_260;
+      } else {
         ntp(c, tout);
         finish(c, OUT_OK);
-        return;
       }
-      default:
-        finish(c, c.outcome);
-        return;
+    }
+  }
+  // ---- 6: DDRIV3 label 260: weights and guards, DDSTP entry (slatec/ddriv3.f90:1292-1351)
+  if (wave_any(at(c, PC_260)) && at(c, PC_260)) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.ywt[i] = dmax(k.ewt, fabs(c.y[i]));
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.save2[i] = c.y[i] / c.ywt[i];
+    double sum = dnrm2(c.save2) / k.sqrtn;
+    sum = dmax(1.0, sum);
+    if (k.eps < sum * k.uround) {
+      copy_yh0_to_y(c);
+      finish(c, OUT_EPS);
+    } else if (c.nstep == k.mxstep) {
+      copy_yh0_to_y(c);
+      finish(c, OUT_MXSTEP);
+    } else {
+      c.told = c.t;                                    // DDSTP entry (:91-106)
+      c.ntry = 0;
+      c.nfail = 0;
+      if (c.jtask <= 0) {
+        ntl_head(c);
+        c.ntl_restart = false;
+        request_f(c, PC_NTL_F);
+      } else {
+        c.pc = PC_ATTEMPT;
+      }
+    }
+  }
+  // ---- 7: attempt, labels 100/110 (slatec/ddstp.f90:107-118)
+  if (wave_any(at(c, PC_ATTEMPT)) && at(c, PC_ATTEMPT)) {
+    c.ntry++;
+    if (c.ntry > 50) {
+      finish(c, OUT_REPEATED_FAIL);
+    } else {
+      c.natt++;
+      c.t = c.t + c.h;
+      psc<true>(c.nq, c.yh);
+      c.evaljc = (fabs(c.rc - 1.0) > 0.3) || (c.nstep >= c.jstepl + 10);
+      c.iter = 0;
+      copy_yh0_to_y(c);
+      request_f(c, PC_PRED_F);
     }
   }
 }
