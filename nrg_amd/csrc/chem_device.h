@@ -69,13 +69,18 @@ struct BdfConst {
 NRG_DI double dmax(double a, double b) { return (a >= b) ? a : b; }
 NRG_DI double dmin(double a, double b) { return (a <= b) ? a : b; }
 
-// One out-of-line copy of each transcendental: the rate constants call them
-// ~120 times per cell and the step-size choice a few times per step; inlined,
-// every call site would carry its own 100+-instruction expansion.
+// One out-of-line copy of each transcendental for the integrator kernel,
+// which calls pow a few times per step from several sites; inlined, every
+// call site would carry its own 100+-instruction expansion.
 __device__ __noinline__ double nl_pow(double x, double y) { return pow(x, y); }
-__device__ __noinline__ double nl_exp(double x) { return exp(x); }
-__device__ __noinline__ double nl_log(double x) { return log(x); }
-__device__ __noinline__ double nl_log10(double x) { return log10(x); }
+
+// Math for the rate-constant kernel, where inlining is the better trade.
+struct InlineMath {
+  static NRG_DI double pow_(double x, double y) { return pow(x, y); }
+  static NRG_DI double exp_(double x) { return exp(x); }
+  static NRG_DI double log_(double x) { return log(x); }
+  static NRG_DI double log10_(double x) { return log10(x); }
+};
 
 // A value pinned in a VGPR.  Select chains over register arrays go through it,
 // so the optimiser cannot turn "select between loads" into one load through a
@@ -147,6 +152,7 @@ NRG_DI double h_molar(const MechCoef &mc, double T, int s)
   return hs * temp;
 }
 
+template <class M = InlineMath>
 NRG_DI double s_molar(const MechCoef &mc, double T, int s)
 {
   const double temp = clip_t(T);
@@ -157,7 +163,7 @@ NRG_DI double s_molar(const MechCoef &mc, double T, int s)
   sm = sm * temp + d[1];
   sm = sm * temp + d[2];
   sm = sm * temp + a[1];
-  return (sm * temp + a[6]) + a[0] * nl_log(temp);
+  return (sm * temp + a[6]) + a[0] * M::log_(temp);
 }
 
 NRG_DI double cp_molar(const MechCoef &mc, double T, int s)
@@ -178,18 +184,19 @@ NRG_DI double e_molar(const MechCoef &mc, double T, int s)
 }
 
 // ---------------------------------------------------------------------------
-// Rate constants (chemical_kinetics_solver.f90:360-493) into the lane's LDS
-// slot: kl[r*WAVE] = k_f(r), kl[(R+r)*WAVE] = k_r(r).  [M] of the reactions
-// whose RHS term uses it go to Mc (:235-248).
+// Rate constants (chemical_kinetics_solver.f90:360-493) of one cell:
+// kout[r*stride] = k_f(r), kout[(R+r)*stride] = k_r(r), and [M] of the
+// reactions whose RHS term uses it, kout[(2R+slot)*stride] (:235-248).
 // ---------------------------------------------------------------------------
-NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], double *kl,
-                           double (&Mc)[NM])
+template <class M>
+NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], double *kout,
+                           long long stride)
 {
   const double T = dmin(Tin, 10000.0);
   double sv[S], hv[S];
 #pragma unroll
   for (int c = 0; c < S; ++c) {
-    sv[c] = s_molar(mc, T, c);
+    sv[c] = s_molar<M>(mc, T, c);
     hv[c] = h_molar(mc, T, c);
   }
 #pragma unroll
@@ -198,7 +205,7 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
     if (uses_m(r)) {
 #pragma unroll
       for (int s = 0; s < S; ++s) m = m + C[s] * mc.eff[r][s];
-      if (kMSlot.v[r] >= 0) Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] = m;
+      if (kMSlot.v[r] >= 0) kout[(2 * R + kMSlot.v[r]) * stride] = m;
     }
     double d_s = 0.0, d_h = 0.0;
 #pragma unroll
@@ -221,43 +228,43 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
     const int rt = topo::rtype[r];
     double kf = 0.0, kr = 0.0;
     if (rt == 0 || rt == 1 || rt == 5 || rt == 6) {
-      kf = (mc.A[r] * nl_pow(T, mc.beta[r])) * nl_exp(mc.mE_R[r] / T);
+      kf = (mc.A[r] * M::pow_(T, mc.beta[r])) * M::exp_(mc.mE_R[r] / T);
     } else if (rt == 2 || rt == 7) {
-      const double k0 = (mc.A_low[r] * nl_pow(T, mc.beta_low[r])) * nl_exp(mc.mElow_R[r] / T);
-      const double kinf = (mc.A[r] * nl_pow(T, mc.beta[r])) * nl_exp(mc.mE_R[r] / T);
+      const double k0 = (mc.A_low[r] * M::pow_(T, mc.beta_low[r])) * M::exp_(mc.mElow_R[r] / T);
+      const double kinf = (mc.A[r] * M::pow_(T, mc.beta[r])) * M::exp_(mc.mE_R[r] / T);
       const double pr = (k0 * m) / kinf;
       kf = kinf * (pr / (1.0 + pr));
     } else if (rt == 3) {
-      double k0 = nl_exp(mc.mElow_R[r] / T);
-      double kinf = nl_exp(mc.mE_R[r] / T);
-      k0 = (mc.A_low[r] * nl_pow(T, mc.beta_low[r])) * k0;
-      kinf = (mc.A[r] * nl_pow(T, mc.beta[r])) * kinf;
+      double k0 = M::exp_(mc.mElow_R[r] / T);
+      double kinf = M::exp_(mc.mE_R[r] / T);
+      k0 = (mc.A_low[r] * M::pow_(T, mc.beta_low[r])) * k0;
+      kinf = (mc.A[r] * M::pow_(T, mc.beta[r])) * kinf;
       const double pr = (k0 * m) / kinf;
-      const double fcent = (mc.one_m_alpha[r] * nl_exp(-(T / mc.troe[1][r])) +
-                            mc.troe[0][r] * nl_exp(-(T / mc.troe[2][r]))) +
-                           nl_exp(-(mc.troe[3][r] / T));
-      const double lf = nl_log10(fcent);
+      const double fcent = (mc.one_m_alpha[r] * M::exp_(-(T / mc.troe[1][r])) +
+                            mc.troe[0][r] * M::exp_(-(T / mc.troe[2][r]))) +
+                           M::exp_(-(mc.troe[3][r] / T));
+      const double lf = M::log10_(fcent);
       const double c = -0.4 - 0.67 * lf;
       const double n = 0.75 - 1.27 * lf;
       const double d = 0.14;
       double blend;
       if (pr != 0.0) {
-        const double lp = nl_log10(pr);
+        const double lp = M::log10_(pr);
         const double x = (lp + c) / (n - d * (lp + c));
-        blend = nl_pow(fcent, 1.0 / (1.0 + x * x));
+        blend = M::pow_(fcent, 1.0 / (1.0 + x * x));
       } else {
-        blend = nl_pow(fcent, -1.0 / d);
+        blend = M::pow_(fcent, -1.0 / d);
       }
       kf = (kinf * (pr / (1.0 + pr))) * blend;
     }
     if (rt == 0 || rt == 1 || rt == 2 || rt == 3) {
-      const double Kp = nl_exp(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
-      // K_c = K_p * ((P_atm/R_u)/T)**d_nu; d_nu = 0 gives nl_pow(x, 0) = 1 exactly
-      const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * nl_pow(mc.patm_r / T, d_nu);
+      const double Kp = M::exp_(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
+      // K_c = K_p * ((P_atm/R_u)/T)**d_nu; d_nu = 0 gives pow(x, 0) = 1 exactly
+      const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * M::pow_(mc.patm_r / T, d_nu);
       kr = kf / Kc;
     }
-    kl[r * WAVE] = kf;
-    kl[(R + r) * WAVE] = kr;
+    kout[r * stride] = kf;
+    kout[(R + r) * stride] = kr;
   }
 }
 

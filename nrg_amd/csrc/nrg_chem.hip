@@ -69,8 +69,11 @@ struct StepArgs {
   int32_t *status;
   int32_t *nstep, *nfe, *nje;          // optional per-cell work counters
   unsigned long long *acc;              // [16] work/status accumulators
-  unsigned long long *queue;            // next unclaimed inner cell (persistent form)
+  unsigned long long *queue;            // next unclaimed cell of the chunk (persistent form)
   int jac_pct;                          // run a Jacobian batch when >= this % of pending lanes want one
+  const double *rates;                  // [2R+NM][nchunk] k_f, k_r, [M] of the chunk's cells
+  double *rates_out;                    // same buffer, written by rates_kernel
+  long long base, nchunk;               // this launch covers inner cells [base, base+nchunk)
 };
 
 // Block-wide staging of the mechanism coefficients into LDS.
@@ -99,7 +102,10 @@ NRG_DI int wave_max_i32(int v)
 
 // accumulator slots
 enum { ACC_INTEG = 0, ACC_NSTEP, ACC_NFE, ACC_NJE, ACC_NCORR, ACC_NATT, ACC_NSTEP_MAX, ACC_NFE_MAX,
-       ACC_STATUS0 };   // ACC_STATUS0 + NRG_CELL_* (8 slots)
+       ACC_STATUS0,     // ACC_STATUS0 + NRG_CELL_* (8 slots)
+       ACC_QUEUE0 = 16, // one queue head per chunk
+       MAX_CHUNKS = 48,
+       ACC_SLOTS = ACC_QUEUE0 + MAX_CHUNKS };
 
 // Zero the work accumulators and the queue head (a kernel, so the launch
 // sequence can be captured in a graph without memset nodes).
@@ -132,9 +138,34 @@ NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long q, long 
   const double rho = a.rho[off];
 #pragma unroll
   for (int s = 0; s < S; ++s) c.y[s] = (a.Y[s][off] * rho) / mc.W[s];
-  rate_constants(mc, T, c.y, kl, Mc);
+  const double *rk = a.rates + (q - a.base);
+#pragma unroll
+  for (int r = 0; r < 2 * R; ++r) kl[r * WAVE] = rk[r * a.nchunk];
+#pragma unroll
+  for (int i = 0; i < NM; ++i) Mc[i] = rk[(2 * R + i) * a.nchunk];
   bdf_init(c, a.dt, a.k);
   return true;
+}
+
+// Rate constants and [M] of every integrated cell of the chunk, one cell per
+// thread (chemical_kinetics_solver.f90:235-252, 360-493).  Kept apart from the
+// integrator so its ~120 transcendentals run coherently at full occupancy.
+__global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
+{
+  __shared__ MechCoef mc;
+  stage_mech(mc, a.mech);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nchunk) return;
+  const long long q = a.base + t;
+  const long long off = cell_offset(a.g, q);
+  const double T = a.T[off];
+  const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
+  if (!(T >= a.k.t_gate && bcv == 0)) return;
+  const double rho = a.rho[off];
+  double C[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) / mc.W[s];
+  rate_constants<InlineMath>(mc, T, C, a.rates_out + t, a.nchunk);
 }
 
 // Output assembly (:318-340) once the cell's DDRIV3 has returned.
@@ -202,8 +233,9 @@ __global__ void __launch_bounds__(WAVE) chem_bdf_kernel(StepArgs a)
   unsigned long long work[6] = {0, 0, 0, 0, 0, 0};
   int mx_step = 0, mx_fe = 0;
   if (!PERSISTENT) {
-    q = (long long)blockIdx.x * WAVE + lane;
-    if (q < a.g.ninner) have = start_cell(a, mc, q, off, c, kl, Mc, cnt);
+    const long long t = (long long)blockIdx.x * WAVE + lane;
+    q = a.base + t;
+    if (t < a.nchunk) have = start_cell(a, mc, q, off, c, kl, Mc, cnt);
   }
   for (;;) {
     if (PERSISTENT) {
@@ -215,8 +247,10 @@ __global__ void __launch_bounds__(WAVE) chem_bdf_kernel(StepArgs a)
         if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)__popcll(mask));
         base = __shfl(base, leader, WAVE);
         if (need) {
-          q = (long long)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
-          if (q >= a.g.ninner)
+          const long long t =
+              (long long)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
+          q = a.base + t;
+          if (t >= a.nchunk)
             exhausted = true;
           else
             have = start_cell(a, mc, q, off, c, kl, Mc, cnt);
@@ -412,6 +446,10 @@ struct nrg_chem_handle {
   int persistent_blocks = 0;               // resident one-wave blocks of the persistent form
   // staging buffers for the host-pointer entry point
   std::vector<void *> stage;
+  // rate-constant buffer of one chunk: [2R+NM][chunk] doubles
+  double *rates = nullptr;
+  long long rates_cells = 0;
+  long long chunk_max = 1ll << 23;
   size_t stage_cells = 0;
 };
 
@@ -648,7 +686,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
   if (hipStreamCreate(&h->stream) != hipSuccess) return fail(set_error(NRG_CHEM_E_HIP, "hipStreamCreate"));
   if (hipMalloc(&h->dev_mech, sizeof(MechCoef)) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipMalloc mech"));
-  if (hipMalloc(&h->dev_acc, 17 * sizeof(unsigned long long)) != hipSuccess)
+  if (hipMalloc(&h->dev_acc, ACC_SLOTS * sizeof(unsigned long long)) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipMalloc acc"));
   {
     hipDeviceProp_t prop;
@@ -678,6 +716,7 @@ void nrg_chem_destroy(nrg_chem_t *h)
     if (p) hipFree(p);
   if (h->dev_mech) hipFree(h->dev_mech);
   if (h->dev_acc) hipFree(h->dev_acc);
+  if (h->rates) hipFree(h->rates);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -717,22 +756,42 @@ static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const d
   a.nfe = nfe;
   a.nje = nje;
   a.acc = h->dev_acc;
-  a.queue = h->dev_acc + 16;
   a.jac_pct = h->jac_pct;
   HIP_TRY(hipSetDevice(h->device));
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, h->stream, h->dev_acc, 17);
+  // cells per chunk: at most chunk_max, and few enough chunks for the queue slots
+  long long chunk = std::min<long long>(a.g.ninner, h->chunk_max);
+  const long long min_chunk = (a.g.ninner + MAX_CHUNKS - 1) / MAX_CHUNKS;
+  chunk = std::max<long long>(std::max<long long>(chunk, min_chunk), 1);
+  const long long nchunks = a.g.ninner > 0 ? (a.g.ninner + chunk - 1) / chunk : 0;
+  if (a.g.ninner > 0 && h->rates_cells < chunk) {
+    if (h->rates) HIP_TRY(hipFree(h->rates));
+    h->rates = nullptr;
+    h->rates_cells = 0;
+    HIP_TRY(hipMalloc(&h->rates, (size_t)chunk * (2 * R + NM) * sizeof(double)));
+    h->rates_cells = chunk;
+  }
+  a.rates = h->rates;
+  a.rates_out = h->rates;
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
   HIP_TRY(hipGetLastError());
   unsigned long long acc[16] = {};
   float ms = 0.f;
   if (a.g.ninner > 0) {
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    if (h->form == NRG_FORM_PERSISTENT) {
-      const long long need = (a.g.ninner + WAVE - 1) / WAVE;
-      const long long nb = std::min<long long>(h->persistent_blocks, need);
-      hipLaunchKernelGGL(chem_bdf_kernel<true>, dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
-    } else {
-      const long long nb = (a.g.ninner + WAVE - 1) / WAVE;
-      hipLaunchKernelGGL(chem_bdf_kernel<false>, dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
+    for (long long ci = 0; ci < nchunks; ++ci) {
+      a.base = ci * chunk;
+      a.nchunk = std::min<long long>(chunk, a.g.ninner - a.base);
+      a.queue = h->dev_acc + ACC_QUEUE0 + ci;
+      hipLaunchKernelGGL(rates_kernel, dim3((unsigned)((a.nchunk + 255) / 256)), dim3(256), 0,
+                         h->stream, a);
+      const long long need = (a.nchunk + WAVE - 1) / WAVE;
+      if (h->form == NRG_FORM_PERSISTENT) {
+        const long long nb = std::min<long long>(h->persistent_blocks, need);
+        hipLaunchKernelGGL(chem_bdf_kernel<true>, dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
+      } else {
+        hipLaunchKernelGGL(chem_bdf_kernel<false>, dim3((unsigned)need), dim3(WAVE), 0, h->stream,
+                           a);
+      }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
