@@ -10,23 +10,30 @@
  * glibc libm, as the reference is built).  Citations are to
  * yakovenko-ivan/NRG; "sl/" abbreviates computing_module/src/third_party/slatec/.
  *
- * Parity pinning (what checks each part; see tests/test_oracle_*.py):
+ * Parity pinning (what checks each part; tests/test_oracle_slatec.py,
+ * tests/test_oracle_chem.py, tests/test_mechanism.py):
  *   or_dnrm2, or_dgefa, or_dgesl, or_ddcst, or_ddriv3
  *       bit-for-bit against SLATEC itself, compiled from the reference's own
- *       sources into oracle/_ref (oracle/Makefile), on random inputs, on
- *       Robertson and linear test problems, and on KEROMNES cells;
- *       plus closed forms (Gear coefficients, numpy norms/LU).
- *   or_h_molar / or_s_molar / or_cp_molar
- *       JANAF/NIST-JANAF standard values (H298, S298, cp(1000 K)) and
- *       continuity at the 1000 K switch; mpmath 50-digit re-evaluation.
- *   or_rate_constants, or_rhs
- *       element (H, O, N) conservation of omega, detailed balance
- *       (k_f/k_r = K_c from an independent equilibrium), Arrhenius and
- *       fall-off limits, mpmath re-evaluation.
- *   or_chem_cell (composition of the above)
- *       scipy BDF/Radau at rtol 1e-13 on the same RHS (agreement ~1e-8).
- *   or_calculate_temperature, or_closure
- *       bisection on e(T) to 1e-12 K; round trip T -> E_f -> T.
+ *       sources into oracle/_ref (oracle/Makefile): DNRM2 through all four
+ *       of Lawson's phases, LU with exact zeros and singular matrices, the
+ *       Gear tables, DDRIV3 on linear, Robertson (incl. MXSTEP exhaustion)
+ *       and KEROMNES cells (identical y, NSTEP, NFE, NJE); plus closed forms
+ *       (Euclidean norm, numpy solve, BDF l-vectors from prod(1 + x/i),
+ *       expm of a stiff linear system, Robertson's known solution).
+ *   or_h_molar / or_s_molar / or_cp_molar / or_e_molar
+ *       NIST-JANAF values at 298.15, 1000 and 2000 K; cp = dh/dT,
+ *       cp/T = ds/dT, e = h - RT; continuity at the 1000 K switch.
+ *   or_mixture_concentration, or_rate_constants, or_rhs
+ *       50-digit mpmath evaluation of the textbook forms (Arrhenius, Troe
+ *       fall-off, K_c from NASA-7 Gibbs energies with dnu over species) for
+ *       all 24 reactions at 700-3100 K; law of mass action; atom balance.
+ *   or_chem_cell / or_chem_cells (composition of the above)
+ *       scipy Radau at rtol 1e-12 on the same frozen-T system (1e-6 rel);
+ *       gating (T < t_gate, bc != 0); atom and mass balance of the outputs.
+ *   or_calculate_temperature, or_closure_cells, or_energy_from_temperature_cells
+ *       bisection on e(T) (1e-6 K); round trip T -> E_f -> T; conservation of
+ *       the mixture internal energy sum_s C_s e_s(T) over chemistry + closure
+ *       (mass-consistent molar masses); p = rho R T / W.
  * Parity with the reference's Fortran chemistry module itself is unpinned by
  * execution: it does not compile with the available Fortran compiler without
  * source edits (namelist/derived-type name clash), see oracle/Makefile.

@@ -1,0 +1,239 @@
+"""The oracle's SLATEC restatement against SLATEC itself and against mathematics.
+
+oracle/_ref/libslatec_ref.so is the reference's own SLATEC
+(computing_module/src/third_party/slatec/*.f90) compiled unmodified by
+oracle/Makefile; every comparison with it is bit-for-bit.  Where it is absent
+(a tree without the reference sources) those tests skip and the closed-form
+checks below still run.
+"""
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import REF_SO, RefSlatec
+
+HMAX32 = float(np.float32(1.0e-7))
+
+needs_ref = pytest.mark.skipif(not os.path.exists(REF_SO),
+                               reason="SLATEC reference build (oracle/_ref) not present")
+
+
+@pytest.fixture(scope="module")
+def ref():
+    return RefSlatec()
+
+
+def _dnrm2_cases():
+    """Vectors that drive Lawson's DNRM2 through each of its four phases
+    (zeros, small < CUTLO = 8.232e-11, mid, large >= CUTHI/n), and switches
+    between them in both directions."""
+    rng = np.random.default_rng(7)
+    c = {
+        "empty": np.zeros(0),
+        "one": np.array([-3.0]),
+        "zeros": np.zeros(9),
+        "mid": rng.uniform(-2, 2, 9),
+        "tiny": rng.uniform(1e-16, 1e-12, 9),
+        "subnormal": np.array([5e-324, 1e-310, 0.0, 2e-320]),
+        "zeros_then_tiny_then_mid": np.array([0.0, 0.0, 1e-13, -4e-12, 0.5, 1e-14, 2.0]),
+        "mid_then_tiny": np.array([1.0, 1e-20, 3.0, -1e-30]),
+        "tiny_then_huge": np.array([1e-15, 1e-14, 3e20, 1e-12]),
+        "mid_then_huge": np.array([1.0, -2.0, 5e19, 3.0, 1e18]),
+        "huge": rng.uniform(1e150, 1e160, 9),
+        "chem_weights": np.abs(rng.standard_normal(9)) * np.logspace(-12, 2, 9),
+    }
+    for k in range(20):
+        n = int(rng.integers(1, 40))
+        v = rng.standard_normal(n) * 10.0 ** rng.uniform(-25, 25, n)
+        v[rng.random(n) < 0.2] = 0.0
+        c["random_%d" % k] = v
+    return c
+
+
+DNRM2 = _dnrm2_cases()
+
+
+@pytest.mark.parametrize("name", sorted(DNRM2))
+def test_dnrm2_matches_euclidean_norm(oracle, name):
+    x = DNRM2[name]
+    want = float(np.sqrt(np.sum((x.astype(np.longdouble)) ** 2))) if len(x) else 0.0
+    got = oracle.dnrm2(x)
+    assert got == pytest.approx(want, rel=4e-15, abs=0.0)
+
+
+@needs_ref
+@pytest.mark.parametrize("name", sorted(DNRM2))
+def test_dnrm2_bitwise_vs_slatec(oracle, ref, name):
+    x = DNRM2[name]
+    assert oracle.dnrm2(x) == ref.dnrm2(x)
+
+
+def _matrices():
+    rng = np.random.default_rng(11)
+    ms = []
+    for k in range(12):
+        a = rng.standard_normal((9, 9)) * 10.0 ** rng.uniform(-3, 3, (9, 9))
+        a[rng.random((9, 9)) < 0.3] = 0.0         # exact zeros: DAXPY's da == 0 skip
+        a += np.diag(rng.uniform(0.5, 2.0, 9))
+        ms.append(a)
+    # chemistry-like: I - h*el0*J with a stiff J
+    J = rng.standard_normal((9, 9)) * 1e6
+    ms.append(np.eye(9) - 1e-7 * J)
+    return ms
+
+
+MATS = _matrices()
+
+
+@pytest.mark.parametrize("k", range(len(MATS)))
+def test_dgefa_dgesl_solve(oracle, k):
+    a = MATS[k]
+    lu, ip, info = oracle.dgefa(a)
+    assert info == 0
+    b = np.arange(1.0, 10.0)
+    x = oracle.dgesl(lu, ip, b)
+    assert np.allclose(a @ x, b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
+    assert np.allclose(x, np.linalg.solve(a, b), rtol=1e-8, atol=1e-12)
+
+
+def test_dgefa_partial_pivoting_is_max_abs(oracle):
+    """Pivot k is the first index of max |a(k:n, k)| of the partly reduced
+    matrix (IDAMAX); here column 0 has its max in row 3."""
+    a = np.eye(4)
+    a[:, 0] = [1.0, -2.0, 0.5, -7.0]
+    _, ip, info = oracle.dgefa(a)
+    assert info == 0 and ip[0] == 3
+
+
+def test_dgefa_singular_info(oracle):
+    """INFO is overwritten at each zero pivot and by the final U(n,n) test, so
+    it names the LAST zero pivot (1-based): ones(3,3) has zero pivots 2 and 3."""
+    _, _, info = oracle.dgefa(np.ones((3, 3)))
+    assert info == 3
+    a = np.diag([1.0, 0.0, 2.0])
+    _, _, info = oracle.dgefa(a)
+    assert info == 2
+
+
+@needs_ref
+def test_dgefa_singular_info_vs_slatec(oracle, ref):
+    for a in (np.ones((3, 3)), np.diag([1.0, 0.0, 2.0]), np.zeros((4, 4))):
+        assert oracle.dgefa(a)[2] == ref.dgefa(a)[2]
+
+
+@needs_ref
+@pytest.mark.parametrize("k", range(len(MATS)))
+def test_dgefa_dgesl_bitwise_vs_slatec(oracle, ref, k):
+    a = MATS[k]
+    lu0, ip0, i0 = oracle.dgefa(a)
+    lu1, ip1, i1 = ref.dgefa(a)
+    assert i0 == i1
+    assert np.array_equal(ip0, ip1)
+    assert np.array_equal(lu0.view(np.int64), lu1.view(np.int64))
+    b = np.linspace(-3.0, 5.0, 9)
+    assert np.array_equal(oracle.dgesl(lu0, ip0, b).view(np.int64),
+                          ref.dgesl(lu1, ip1, b).view(np.int64))
+
+
+def test_ddcst_bdf_coefficients_closed_form(oracle):
+    """Gear's BDF l-vector of order q: coefficients of prod_{i=1..q} (1 + x/i),
+    normalised so that l_1 = 1 (EL(2,q) = 1)."""
+    el, tq = oracle.ddcst(5)
+    for q in range(1, 6):
+        poly = [Fraction(1)]
+        for i in range(1, q + 1):
+            nxt = [Fraction(0)] * (len(poly) + 1)
+            for d, c in enumerate(poly):
+                nxt[d] += c
+                nxt[d + 1] += c / i
+            poly = nxt
+        want = [float(c / poly[1]) for c in poly]
+        got = el[q - 1, :q + 1]
+        # DDCST builds the table by a floating-point recurrence: a few ulps
+        assert np.allclose(got, want, rtol=9e-16, atol=0), (q, got - np.array(want))
+        assert np.all(el[q - 1, q + 1:] == 0)
+
+
+@needs_ref
+def test_ddcst_bitwise_vs_slatec(oracle, ref):
+    el0, tq0 = oracle.ddcst(5)
+    el1, tq1 = ref.ddcst(5)
+    assert np.array_equal(el0[:5], el1[:5])
+    assert np.array_equal(tq0[:5], tq1[:5])
+
+
+# ---- DDRIV3 -------------------------------------------------------------------------
+
+A_LIN = np.array([[-1.0e4, 1.0e2, 0.0], [1.0e4, -1.0e2 - 1.0, 3.0], [0.0, 1.0, -3.0]])
+
+
+def lin_rhs(t, y):
+    return A_LIN @ y
+
+
+def robertson(t, y):
+    return np.array([-0.04 * y[0] + 1e4 * y[1] * y[2],
+                     0.04 * y[0] - 1e4 * y[1] * y[2] - 3e7 * y[1] ** 2,
+                     3e7 * y[1] ** 2])
+
+
+def test_ddriv3_linear_against_matrix_exponential(oracle):
+    from scipy.linalg import expm
+    y0 = np.array([1.0, 0.5, 0.25])
+    for tout in (1e-3, 0.1):
+        y, out = oracle.ddriv3(lin_rhs, y0, tout, eps=1e-9, ewt=1e-4, hmax=1.0)
+        assert out.nstate == 2 and out.ierflg == 0
+        want = expm(A_LIN * tout) @ y0
+        assert np.allclose(y, want, rtol=1e-6, atol=1e-8), (tout, y, want)
+
+
+def test_ddriv3_robertson_conserves_mass(oracle):
+    y0 = np.array([1.0, 0.0, 0.0])
+    y, out = oracle.ddriv3(robertson, y0, 40.0, eps=1e-8, ewt=1e-6, hmax=10.0)
+    assert out.nstate == 2
+    assert abs(y.sum() - 1.0) < 1e-9
+    # known solution at t = 40 (Hairer & Wanner, II.10)
+    assert y[0] == pytest.approx(0.7158, abs=2e-4)
+    assert y[2] == pytest.approx(0.2842, abs=2e-4)
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["linear_short", "linear_long", "robertson",
+                                  "robertson_mxstep", "linear_hmax32"])
+def test_ddriv3_bitwise_vs_slatec(oracle, ref, case):
+    f, y0, tout, kw = {
+        "linear_short": (lin_rhs, [1.0, 0.5, 0.25], 1e-3, dict(hmax=1.0)),
+        "linear_long": (lin_rhs, [1.0, 0.5, 0.25], 0.3, dict(hmax=1.0, eps=1e-6)),
+        "robertson": (robertson, [1.0, 0.0, 0.0], 4.0, dict(hmax=10.0, eps=1e-7, ewt=1e-6)),
+        "robertson_mxstep": (robertson, [1.0, 0.0, 0.0], 4.0, dict(hmax=10.0, mxstep=25)),
+        "linear_hmax32": (lin_rhs, [1.0, 0.5, 0.25], 2e-6, dict(hmax=HMAX32)),
+    }[case]
+    y0 = np.array(y0)
+    y_or, out = oracle.ddriv3(f, y0, tout, **kw)
+    y_rf, info = ref.ddriv3(f, y0, tout, **kw)
+    assert out.nstate == info["nstate"] and out.ierflg == info["ierflg"]
+    assert (out.nstep, out.nfe, out.nje) == (info["nstep"], info["nfe"], info["nje"])
+    assert np.array_equal(y_or.view(np.int64), y_rf.view(np.int64)), (y_or, y_rf)
+
+
+def _chem_cells(mech, n, seed):
+    from nrg_amd import inputs
+    st = inputs.to_numpy(inputs.ignition_mix(mech, n, seed=seed, frac_window=0.5))
+    return [(float(st["T"][i]), float(st["rho"][i]), np.ascontiguousarray(st["Y"][:, i]))
+            for i in range(n)]
+
+
+@needs_ref
+@pytest.mark.parametrize("dt", [1e-8, 1e-7, 1e-6])
+def test_ddriv3_keromnes_cells_bitwise_vs_slatec(mech, oracle, ref, dt):
+    """The chemistry right-hand side of the oracle integrated by the oracle's
+    DDRIV3 and by SLATEC's: identical y, NSTEP, NFE, NJE."""
+    for T, rho, Y in _chem_cells(mech, 12, seed=3):
+        y_or, cin, out = oracle.ddriv3_cell(T, rho, Y, dt)
+        y_rf, cin2, info = ref.ddriv3_cell(oracle, T, rho, Y, dt)
+        assert np.array_equal(cin, cin2)
+        assert out.nstate == info["nstate"]
+        assert (out.nstep, out.nfe, out.nje) == (info["nstep"], info["nfe"], info["nje"])
+        assert np.array_equal(y_or.view(np.int64), y_rf.view(np.int64)), (T, y_or - y_rf)
