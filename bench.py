@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
     ap.add_argument("--form", choices=["plain", "persistent"], default="persistent",
                     help="integrator launch form (same results)")
+    ap.add_argument("--order", choices=["natural", "stiffness"], default="natural",
+                    help="queue order of the cells (same results)")
     ap.add_argument("--jac-pct", type=int, default=0,
                     help="batched-Jacobian threshold in percent of pending lanes (0: default)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -80,7 +82,9 @@ def main():
     for s in range(S):
         Yf.pr[s].inner(g).copy_(st["Y"][s].view(view))
     form = capi.NRG_FORM_PERSISTENT if args.form == "persistent" else capi.NRG_FORM_PLAIN
-    solver = chemical_kinetics_solver_c(dm, capi.default_opts(form=form, jac_batch_pct=args.jac_pct))
+    order = capi.NRG_ORDER_STIFFNESS if args.order == "stiffness" else capi.NRG_ORDER_NATURAL
+    solver = chemical_kinetics_solver_c(dm, capi.default_opts(form=form, jac_batch_pct=args.jac_pct,
+                                                              cell_order=order))
     E_f = dm.fields["full_energy"]
     solver.engine.energy_from_temperature_dev(g, dm.fields["temperature"].ptr(), Yf.ptrs(),
                                               E_f.ptr())
@@ -201,7 +205,7 @@ def main():
             "workload": "FP64 Gear-BDF split step (SLATEC DDRIV3 semantics, FD Jacobian), "
                         "256^3 hot kernels in fresh H2-air, KEROMNES 9sp/24r, dt=1e-7 s",
             "cells": ncells, "dt": args.dt, "mechanism": "KEROMNES",
-            "path": "fp64_integrator", "form": args.form,
+            "path": "fp64_integrator", "form": args.form, "order": args.order,
         },
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,

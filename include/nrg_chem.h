@@ -66,6 +66,18 @@ enum {
   NRG_JAC_ANALYTIC = 1         /* analytic d(omega)/dC at fixed k: fast mode                  */
 };
 
+/*
+ * Queue order of the cells (same arithmetic, same results).  STIFFNESS bins
+ * cells by the step count each took in the handle's previous call on the same
+ * grid (by temperature on the first call) and hands out the most expensive
+ * bins first: a wave then holds cells of similar cost, and the long cells
+ * start early instead of forming the tail.
+ */
+enum {
+  NRG_ORDER_NATURAL = 0,       /* inner-loop order                                            */
+  NRG_ORDER_STIFFNESS = 1
+};
+
 /* Launch form of the per-cell integrator (same arithmetic, same results). */
 enum {
   NRG_FORM_PLAIN = 0,          /* one cell per lane, one launch over all inner cells          */
@@ -118,7 +130,8 @@ typedef struct nrg_chem_opts {
   int32_t jac_batch_pct;       /* a wave runs a batched Jacobian when at least this percentage
                                   of its pending lanes wait for one (1..100; 0 = default 50)   */
   int32_t waves_per_cu;        /* persistent form: resident waves per CU (0 = occupancy limit) */
-  int32_t reserved[5];
+  int32_t cell_order;          /* NRG_ORDER_*: order in which the queue hands out cells        */
+  int32_t reserved[4];
 } nrg_chem_opts;
 
 /*

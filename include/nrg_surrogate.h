@@ -1,0 +1,90 @@
+/*
+ * nrg_surrogate.h -- learned chemistry step with a conservation/confidence
+ * gate and FP64 fallback (gfx950).
+ *
+ * No counterpart in the reference, which integrates every cell with SLATEC
+ * (computing_module/src/current_build/chemical_kinetics_solver.f90:175-358).
+ * nrg_chem_surrogate_step_dev() is a drop-in for nrg_chem_step_dev(): same
+ * fields, same outputs (Y_prod, Y_prod2, E_f_prod), so the split-step
+ * closure is unchanged.  Per cell:
+ *   1. gated cells (T < t_gate or bc != 0) are handled as the FP64 step does;
+ *   2. the network (nrg_mlp.h) maps features
+ *        f = (T, ln p, ln(Y_s + y_floor) for s = 1..S, ln dt)
+ *      normalised as x_i = (f_i - in_shift_i) * in_scale_i, to the increments
+ *      of one split step, (dT, dY_1..dY_S)_j = y_j * out_scale_j + out_shift_j;
+ *   3. the gate accepts a cell only if
+ *        - T, p, dt lie inside the trained domain,
+ *        - the raw dY conserves every element to elem_tol (relative to the
+ *          atoms it moves) + elem_floor [mol/kg],
+ *        - after projecting dY onto the element-conserving subspace, no
+ *          mass fraction falls below -y_neg_tol,
+ *        - the network's dT agrees within energy_tol [K] with the temperature
+ *          that energy conservation gives for the projected dY (the same
+ *          source application and e->T inversion as nrg_chem_closure_dev);
+ *      accepted cells get the projected increments as Y_prod, Y_prod2,
+ *      E_f_prod (so chemistry conserves elements and energy exactly);
+ *   4. every rejected cell is integrated by the FP64 Gear-BDF kernels of
+ *      nrg_chem.h, exactly as nrg_chem_step_dev would.
+ */
+#ifndef NRG_SURROGATE_H
+#define NRG_SURROGATE_H
+
+#include <stdint.h>
+
+#include "nrg_chem.h"
+#include "nrg_mlp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NRG_SUR_MAX_IN 16
+#define NRG_SUR_MAX_OUT 16
+#define NRG_SUR_MAX_ELEMENTS 4
+
+/* per-cell status of an accepted surrogate answer (extends NRG_CELL_*) */
+#define NRG_CELL_SURROGATE 6
+
+typedef struct nrg_surrogate_norm {
+  float in_shift[NRG_SUR_MAX_IN], in_scale[NRG_SUR_MAX_IN];     /* S + 3 used */
+  float out_shift[NRG_SUR_MAX_OUT], out_scale[NRG_SUR_MAX_OUT]; /* S + 1 used */
+  float y_floor;
+  int32_t reserved[7];
+} nrg_surrogate_norm;
+
+typedef struct nrg_surrogate_gate {
+  double t_min, t_max, p_min, p_max, dt_min, dt_max;
+  double elem_tol, elem_floor;
+  double y_neg_tol;
+  double energy_tol;
+  int32_t n_elements;                                /* rows of `elements` (<= 4)      */
+  int32_t reserved[7];
+  double elements[NRG_SUR_MAX_ELEMENTS][32];         /* atoms of element e in species s */
+} nrg_surrogate_gate;
+
+typedef struct nrg_surrogate_stats {
+  int64_t cells_inner, cells_gated, cells_accepted, cells_fallback;
+  int64_t reject_domain, reject_elements, reject_negative, reject_energy;
+  double infer_ms;            /* featurise + network, HIP events on the handle's stream */
+  double gate_ms;
+  double fallback_ms;         /* FP64 integration of the rejected cells                */
+  nrg_chem_stats fallback;    /* counters of that integration                          */
+} nrg_surrogate_stats;
+
+/*
+ * One chemistry step with the surrogate.  Arguments as nrg_chem_step_dev, plus
+ * the trained network, its normalisation and the gate; `p` is required (the
+ * network reads ln p).  Returns the FP64 fallback's fatal codes as
+ * nrg_chem_step_dev does.
+ */
+int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surrogate_norm *norm,
+                                const nrg_surrogate_gate *gate, const nrg_grid *grid, double dt,
+                                const double *T, const double *p, const double *rho,
+                                const double *const *Y, const int8_t *bc,
+                                double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                                int32_t *cell_status, nrg_surrogate_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NRG_SURROGATE_H */

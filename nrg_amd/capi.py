@@ -36,6 +36,14 @@ NRG_JAC_ANALYTIC = 1
 NRG_FORM_PLAIN = 0
 NRG_FORM_PERSISTENT = 1
 
+NRG_ORDER_NATURAL = 0
+NRG_ORDER_STIFFNESS = 1
+
+NRG_CELL_SURROGATE = 6          # include/nrg_surrogate.h
+NRG_SUR_MAX_IN = 16
+NRG_SUR_MAX_OUT = 16
+NRG_SUR_MAX_ELEMENTS = 4
+
 
 class NrgChemMech(C.Structure):
     _fields_ = [
@@ -69,7 +77,8 @@ class NrgChemOpts(C.Structure):
         ("form", C.c_int32),
         ("jac_batch_pct", C.c_int32),
         ("waves_per_cu", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("cell_order", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -113,6 +122,33 @@ class NrgChemStats(C.Structure):
         }
 
 
+class NrgSurrogateNorm(C.Structure):
+    """struct nrg_surrogate_norm (include/nrg_surrogate.h)."""
+    _fields_ = [("in_shift", C.c_float * NRG_SUR_MAX_IN), ("in_scale", C.c_float * NRG_SUR_MAX_IN),
+                ("out_shift", C.c_float * NRG_SUR_MAX_OUT),
+                ("out_scale", C.c_float * NRG_SUR_MAX_OUT),
+                ("y_floor", C.c_float), ("reserved", C.c_int32 * 7)]
+
+
+class NrgSurrogateGate(C.Structure):
+    """struct nrg_surrogate_gate (include/nrg_surrogate.h)."""
+    _fields_ = [("t_min", C.c_double), ("t_max", C.c_double), ("p_min", C.c_double),
+                ("p_max", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+                ("elem_tol", C.c_double), ("elem_floor", C.c_double), ("y_neg_tol", C.c_double),
+                ("energy_tol", C.c_double), ("n_elements", C.c_int32), ("reserved", C.c_int32 * 7),
+                ("elements", (C.c_double * 32) * NRG_SUR_MAX_ELEMENTS)]
+
+
+class NrgSurrogateStats(C.Structure):
+    """struct nrg_surrogate_stats (include/nrg_surrogate.h)."""
+    _fields_ = [("cells_inner", C.c_int64), ("cells_gated", C.c_int64),
+                ("cells_accepted", C.c_int64), ("cells_fallback", C.c_int64),
+                ("reject_domain", C.c_int64), ("reject_elements", C.c_int64),
+                ("reject_negative", C.c_int64), ("reject_energy", C.c_int64),
+                ("infer_ms", C.c_double), ("gate_ms", C.c_double), ("fallback_ms", C.c_double),
+                ("fallback", NrgChemStats)]
+
+
 def default_opts(**kw):
     """Reference literals (chemical_kinetics_solver.f90:270-290), see nrg_chem_default_opts."""
     o = NrgChemOpts()
@@ -127,6 +163,7 @@ def default_opts(**kw):
     o.form = NRG_FORM_PLAIN
     o.jac_batch_pct = 0
     o.waves_per_cu = 0
+    o.cell_order = NRG_ORDER_NATURAL
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -216,6 +253,14 @@ def load_library(path=None):
         raise FileNotFoundError(
             "libnrg_chem.so is not built (%s); run __graft_entry__.build() "
             "or `make -C nrg_amd/csrc`" % p)
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Load torch first so this library binds to the runtime torch uses: with
+    # the library loaded first, torch finds ROCm's runtime already in the
+    # process and reports no GPUs.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(p)
     vp = C.c_void_p
     lib.nrg_chem_default_opts.argtypes = [C.POINTER(NrgChemOpts)]
@@ -249,6 +294,31 @@ def load_library(path=None):
     lib.nrg_energy_from_temperature_dev.restype = C.c_int
     lib.nrg_chem_stream.argtypes = [vp]
     lib.nrg_chem_stream.restype = vp
+    # include/nrg_mlp.h
+    lib.nrg_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   C.POINTER(vp)]
+    lib.nrg_mlp_create.restype = C.c_int
+    lib.nrg_mlp_destroy.argtypes = [vp]
+    lib.nrg_mlp_destroy.restype = None
+    lib.nrg_mlp_num_params.argtypes = [vp]
+    lib.nrg_mlp_num_params.restype = C.c_int64
+    lib.nrg_mlp_set_params.argtypes = [vp, vp]
+    lib.nrg_mlp_set_params.restype = C.c_int
+    lib.nrg_mlp_get_params.argtypes = [vp, vp]
+    lib.nrg_mlp_get_params.restype = C.c_int
+    lib.nrg_mlp_forward_dev.argtypes = [vp, C.c_int64, vp, vp]
+    lib.nrg_mlp_forward_dev.restype = C.c_int
+    lib.nrg_mlp_grad_dev.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp]
+    lib.nrg_mlp_grad_dev.restype = C.c_int
+    lib.nrg_mlp_train_step_dev.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_float, vp]
+    lib.nrg_mlp_train_step_dev.restype = C.c_int
+    lib.nrg_mlp_stream.argtypes = [vp]
+    lib.nrg_mlp_stream.restype = vp
+    # include/nrg_surrogate.h
+    lib.nrg_chem_surrogate_step_dev.argtypes = [
+        vp, vp, C.POINTER(NrgSurrogateNorm), C.POINTER(NrgSurrogateGate), C.POINTER(NrgGrid),
+        C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(NrgSurrogateStats)]
+    lib.nrg_chem_surrogate_step_dev.restype = C.c_int
     if path is None:
         _lib = lib
     return lib

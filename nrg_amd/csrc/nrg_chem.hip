@@ -15,11 +15,12 @@
 #include <vector>
 
 #include "../../include/nrg_chem.h"
+#include "../../include/nrg_surrogate.h"
 #include "chem_device.h"
+#include "mlp_internal.h"
+#include "nrg_common.h"
 
-using namespace nrg;
-
-namespace {
+namespace nrg {
 
 thread_local char g_last_error[512] = "";
 
@@ -32,12 +33,11 @@ int set_error(int status, const char *fmt, ...)
   return status;
 }
 
-#define HIP_TRY(expr)                                                                        \
-  do {                                                                                       \
-    hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess)                                                                    \
-      return set_error(NRG_CHEM_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
-  } while (0)
+}  // namespace nrg
+
+using namespace nrg;
+
+namespace {
 
 // Field addressing of one call (Fortran column-major with ghost layers).
 struct GridIdx {
@@ -73,8 +73,16 @@ struct StepArgs {
   int jac_pct;                          // run a Jacobian batch when >= this % of pending lanes want one
   const double *rates;                  // [2R+NM][nchunk] k_f, k_r, [M] of the chunk's cells
   double *rates_out;                    // same buffer, written by rates_kernel
-  long long base, nchunk;               // this launch covers inner cells [base, base+nchunk)
+  long long base, nchunk;               // this launch covers queue positions [base, base+nchunk)
+  const int32_t *list;                  // NULL: position t is inner cell t; else inner cell list[t]
+  int32_t *hist;                        // optional: NSTEP of each inner cell (0 if gated), for ordering
 };
+
+// Inner cell (inner-loop order) at position base + t of the launch's queue.
+NRG_DI long long cell_of(const StepArgs &a, long long t)
+{
+  return a.list ? (long long)a.list[a.base + t] : a.base + t;
+}
 
 // Block-wide staging of the mechanism coefficients into LDS.
 NRG_DI void stage_mech(MechCoef &dst, const MechCoef *src)
@@ -116,8 +124,9 @@ __global__ void reset_kernel(unsigned long long *acc, int n)
 
 // Load one cell and set it up (chemical_kinetics_solver.f90:228-265).
 // Returns false for a gated cell, whose outputs are written here (:341-346).
-NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long q, long long &off, Cell &c,
-                       double *kl, double (&Mc)[NM], unsigned long long (&cnt)[8])
+NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long t, long long q,
+                       long long &off, Cell &c, double *kl, double (&Mc)[NM],
+                       unsigned long long (&cnt)[8])
 {
   off = cell_offset(a.g, q);
   const double T = a.T[off];
@@ -132,13 +141,14 @@ NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long q, long 
       a.nfe[q] = 0;
       a.nje[q] = 0;
     }
+    if (a.hist) a.hist[q] = 0;
     cnt[NRG_CELL_GATED]++;
     return false;
   }
   const double rho = a.rho[off];
 #pragma unroll
   for (int s = 0; s < S; ++s) c.y[s] = (a.Y[s][off] * rho) / mc.W[s];
-  const double *rk = a.rates + (q - a.base);
+  const double *rk = a.rates + t;
 #pragma unroll
   for (int r = 0; r < 2 * R; ++r) kl[r * WAVE] = rk[r * a.nchunk];
 #pragma unroll
@@ -156,7 +166,7 @@ __global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
   stage_mech(mc, a.mech);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nchunk) return;
-  const long long q = a.base + t;
+  const long long q = cell_of(a, t);
   const long long off = cell_offset(a.g, q);
   const double T = a.T[off];
   const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
@@ -195,6 +205,7 @@ NRG_DI void finish_cell(const StepArgs &a, const MechCoef &mc, long long q, long
     a.nfe[q] = c.nfe;
     a.nje[q] = c.nje;
   }
+  if (a.hist) a.hist[q] = c.nstep;
 #pragma unroll
   for (int st = 0; st < 8; ++st) cnt[st] += (status == st) ? 1ull : 0ull;
   work[0] += 1;
@@ -234,8 +245,10 @@ __global__ void __launch_bounds__(WAVE) chem_bdf_kernel(StepArgs a)
   int mx_step = 0, mx_fe = 0;
   if (!PERSISTENT) {
     const long long t = (long long)blockIdx.x * WAVE + lane;
-    q = a.base + t;
-    if (t < a.nchunk) have = start_cell(a, mc, q, off, c, kl, Mc, cnt);
+    if (t < a.nchunk) {
+      q = cell_of(a, t);
+      have = start_cell(a, mc, t, q, off, c, kl, Mc, cnt);
+    }
   }
   for (;;) {
     if (PERSISTENT) {
@@ -249,11 +262,12 @@ __global__ void __launch_bounds__(WAVE) chem_bdf_kernel(StepArgs a)
         if (need) {
           const long long t =
               (long long)(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull)));
-          q = a.base + t;
-          if (t >= a.nchunk)
+          if (t >= a.nchunk) {
             exhausted = true;
-          else
-            have = start_cell(a, mc, q, off, c, kl, Mc, cnt);
+          } else {
+            q = cell_of(a, t);
+            have = start_cell(a, mc, t, q, off, c, kl, Mc, cnt);
+          }
         }
         need = !have && !exhausted;
       }
@@ -427,6 +441,302 @@ __global__ void __launch_bounds__(256) energy_kernel(EnergyArgs a)
   a.E_f[off] = Ef;
 }
 
+// ---------------------------------------------------------------------------
+// Stiffness-ordered queue: bin the inner cells by expected cost, most
+// expensive bin first; within a block the cells keep their relative order.
+// ---------------------------------------------------------------------------
+constexpr int NBIN = 64;
+
+struct BinArgs {
+  GridIdx g;
+  const double *T;
+  const int8_t *bc;
+  double t_gate;
+  const int32_t *hist;              // NSTEP of the previous call, or NULL (use T)
+  unsigned long long *count;        // [NBIN]
+  unsigned long long *cursor;       // [NBIN] first queue position of each bin
+  int32_t *list;                    // [ninner]
+};
+
+// 0 = gated (no integration); otherwise grows with the expected cost.
+NRG_DI int cost_bin(const BinArgs &a, long long q)
+{
+  const long long off = cell_offset(a.g, q);
+  const int8_t b = a.bc ? a.bc[off] : (int8_t)0;
+  const double T = a.T[off];
+  if (!(T >= a.t_gate && b == 0)) return 0;
+  if (a.hist) {
+    const int n = max(a.hist[q], 0);
+    const int k = n <= 31 ? n : 31 + (n - 28) / 4;     // unit bins to 31 steps, then 4-step bins
+    return 1 + min(NBIN - 2, k);
+  }
+  const double x = (T - a.t_gate) * (1.0 / 50.0);                     // first call: 50 K bins
+  return 1 + (x < (double)(NBIN - 2) ? (int)x : NBIN - 2);
+}
+
+__global__ void __launch_bounds__(256) bin_count_kernel(BinArgs a)
+{
+  __shared__ unsigned int h[NBIN];
+  if (threadIdx.x < NBIN) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < a.g.ninner) atomicAdd(&h[cost_bin(a, q)], 1u);
+  __syncthreads();
+  if (threadIdx.x < NBIN && h[threadIdx.x]) atomicAdd(&a.count[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) bin_scatter_kernel(BinArgs a)
+{
+  __shared__ unsigned int h[NBIN];
+  __shared__ unsigned long long base[NBIN];
+  if (threadIdx.x < NBIN) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = q < a.g.ninner ? cost_bin(a, q) : -1;
+  unsigned int rank = 0;
+  if (b >= 0) rank = atomicAdd(&h[b], 1u);
+  __syncthreads();
+  if (threadIdx.x < NBIN && h[threadIdx.x])
+    base[threadIdx.x] = atomicAdd(&a.cursor[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+  __syncthreads();
+  if (b >= 0) a.list[base[b] + rank] = (int32_t)q;
+}
+
+// ---------------------------------------------------------------------------
+// Learned step: features -> network -> gate -> FP64 fallback list
+// ---------------------------------------------------------------------------
+constexpr int NIN = S + 3, NOUT = S + 1;
+
+enum { SUR_FALLBACK = 0, SUR_ACCEPT, SUR_GATED, SUR_R_DOMAIN, SUR_R_ELEM, SUR_R_NEG, SUR_R_ENERGY,
+       SUR_SLOTS = 8 };
+
+struct SurArgs {
+  GridIdx g;
+  double dt, t_gate;
+  const MechCoef *mech;
+  const double *T, *p, *rho;
+  const double *Y[S];
+  const int8_t *bc;
+  float in_shift[NIN], in_scale[NIN], out_shift[NOUT], out_scale[NOUT];
+  float y_floor;
+  double t_min, t_max, p_min, p_max;
+  int dt_ok;
+  double elem_tol, elem_floor, y_neg_tol, energy_tol;
+  int ne;
+  double E[NRG_SUR_MAX_ELEMENTS][S];
+  bf16 *X;                    // ninner x MLP_IN_PAD
+  const float *Ypred;         // ninner x NOUT
+  double *Yp[S], *Yp2[S];
+  double *Efp;
+  int32_t *status;
+  int32_t *list;              // fallback cells (inner index)
+  unsigned long long *acc;    // SUR_* counters
+};
+
+NRG_DI bool gated_cell(const SurArgs &a, long long off)
+{
+  const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
+  return !(a.T[off] >= a.t_gate && bcv == 0);
+}
+
+__global__ void __launch_bounds__(256) featurize_kernel(SurArgs a)
+{
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.g.ninner) return;
+  const long long off = cell_offset(a.g, q);
+  bf16 *x = a.X + q * MLP_IN_PAD;
+  float f[MLP_IN_PAD];
+#pragma unroll
+  for (int i = 0; i < MLP_IN_PAD; ++i) f[i] = 0.0f;
+  if (gated_cell(a, off)) {
+    // the FP64 step's handling of a skipped cell (chemical_kinetics_solver.f90:341-346)
+#pragma unroll
+    for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
+    a.Efp[off] = 0.0;
+    if (a.status) a.status[off] = NRG_CELL_GATED;
+  } else {
+    f[0] = (float)a.T[off];
+    f[1] = (float)log(a.p[off]);
+#pragma unroll
+    for (int s = 0; s < S; ++s) f[2 + s] = logf((float)a.Y[s][off] + a.y_floor);
+    f[S + 2] = (float)log(a.dt);
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) f[i] = (f[i] - a.in_shift[i]) * a.in_scale[i];
+  }
+#pragma unroll
+  for (int i = 0; i < MLP_IN_PAD; ++i) x[i] = (bf16)f[i];
+}
+
+// Solve the n x n system M z = b in place (n <= 4, partial pivoting).
+NRG_DI bool small_solve(double (&M)[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS],
+                        double (&b)[NRG_SUR_MAX_ELEMENTS], int n)
+{
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(M[i][k]) > fabs(M[piv][k])) piv = i;
+    if (M[piv][k] == 0.0) return false;
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) {
+        const double t = M[k][j];
+        M[k][j] = M[piv][j];
+        M[piv][j] = t;
+      }
+      const double t = b[k];
+      b[k] = b[piv];
+      b[piv] = t;
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = M[i][k] / M[k][k];
+      for (int j = k; j < n; ++j) M[i][j] -= f * M[k][j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    double s = b[k];
+    for (int j = k + 1; j < n; ++j) s -= M[k][j] * b[j];
+    b[k] = s / M[k][k];
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
+{
+  __shared__ MechCoef mc;
+  stage_mech(mc, a.mech);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = q < a.g.ninner;
+  const long long off = valid ? cell_offset(a.g, q) : 0;
+  const bool gated = valid && gated_cell(a, off);
+  const bool live = valid && !gated;
+  int reason = 0;   // 0 accept, else SUR_R_*
+  if (live) {
+    const double T = a.T[off], p = a.p[off], rho = a.rho[off];
+    double y[S], dy[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) y[s] = a.Y[s][off];
+    const float *yp = a.Ypred + q * NOUT;
+    const double dT = (double)yp[0] * a.out_scale[0] + a.out_shift[0];
+    bool finite = isfinite(dT);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      dy[s] = (double)yp[1 + s] * a.out_scale[1 + s] + a.out_shift[1 + s];
+      finite = finite && isfinite(dy[s]);
+    }
+    if (!finite || !(T >= a.t_min && T <= a.t_max && p >= a.p_min && p <= a.p_max) || !a.dt_ok)
+      reason = SUR_R_DOMAIN;
+    // element balance of the raw increment: sum_s E_es dY_s / W_s  [mol/kg]
+    double res[NRG_SUR_MAX_ELEMENTS] = {0.0, 0.0, 0.0, 0.0};
+    double d[S];
+    if (!reason) {
+      for (int e = 0; e < a.ne; ++e) {
+        double r = 0.0, sc = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          r += a.E[e][s] * dy[s] / mc.W[s];
+          sc += a.E[e][s] * fabs(dy[s]) / mc.W[s];
+        }
+        res[e] = r;
+        if (fabs(r) > a.elem_tol * sc + a.elem_floor) reason = SUR_R_ELEM;
+      }
+    }
+    if (!reason) {
+      // project onto sum_s E_es dY_s / W_s = 0, weighting the correction by |dY_s|
+      double M[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS] = {};
+#pragma unroll
+      for (int s = 0; s < S; ++s) d[s] = fabs(dy[s]) + 1e-12;
+      for (int e = 0; e < a.ne; ++e)
+        for (int f = 0; f < a.ne; ++f) {
+          double m = 0.0;
+#pragma unroll
+          for (int s = 0; s < S; ++s) m += a.E[e][s] * a.E[f][s] * d[s] / (mc.W[s] * mc.W[s]);
+          M[e][f] = m;
+        }
+      if (small_solve(M, res, a.ne)) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          double c = 0.0;
+          for (int e = 0; e < a.ne; ++e) c += a.E[e][s] * res[e];
+          dy[s] -= d[s] * c / mc.W[s];
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (y[s] + dy[s] < -a.y_neg_tol) reason = SUR_R_NEG;
+    }
+    double dc[S], efp = 0.0;
+    if (!reason) {
+      // chemistry sources of the projected increment, as the FP64 step writes them
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        dc[s] = (rho * dy[s]) / mc.W[s];
+        efp = efp - (mc.h_ref[s] * dc[s]) / a.dt;
+      }
+      // energy: E_f(T, Y), source application and e->T, as nrg_chem_closure_dev
+      double avg = 0.0, X[S], e0 = 0.0, hc = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) avg = avg + y[s] / mc.W[s];
+      double wmix = 1.0 / avg;
+#pragma unroll
+      for (int s = 0; s < S; ++s) X[s] = (y[s] / mc.W[s]) * wmix;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        e0 = e0 + e_molar(mc, T, s) * X[s];
+        hc = hc + mc.h_ref[s] * X[s];
+      }
+      const double Ef = (e0 - hc) / wmix + (0.0 + efp * a.dt) / rho;
+      double y1[S], sum = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        y1[s] = y[s] + (0.0 + ((rho * dy[s]) / a.dt) * a.dt) / rho;
+        sum = sum + y1[s];
+      }
+      avg = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        y1[s] = dmax(y1[s], 0.0) / sum;
+        avg = avg + y1[s] / mc.W[s];
+      }
+      wmix = 1.0 / avg;
+#pragma unroll
+      for (int s = 0; s < S; ++s) X[s] = (y1[s] / mc.W[s]) * wmix;
+      const double T1 = calculate_temperature(mc, T, Ef * wmix, X);
+      if (!(fabs(dT - (T1 - T)) <= a.energy_tol)) reason = SUR_R_ENERGY;
+    }
+    if (!reason) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        a.Yp[s][off] = (dc[s] * mc.W[s]) / a.dt;
+        a.Yp2[s][off] = dc[s];
+      }
+      a.Efp[off] = efp;
+      if (a.status) a.status[off] = NRG_CELL_SURROGATE;
+    }
+  }
+  const bool rej = live && reason != 0;
+  const unsigned long long mask = __ballot(rej ? 1 : 0);
+  if (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&a.acc[SUR_FALLBACK], (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader, WAVE);
+    if (rej) a.list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)q;
+  }
+  const unsigned long long n_acc = __popcll(__ballot(live && reason == 0 ? 1 : 0));
+  const unsigned long long n_gat = __popcll(__ballot(gated ? 1 : 0));
+  unsigned long long n_r[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) n_r[i] = __popcll(__ballot(rej && reason == SUR_R_DOMAIN + i ? 1 : 0));
+  if (lane == 0) {
+    if (n_acc) atomicAdd(&a.acc[SUR_ACCEPT], n_acc);
+    if (n_gat) atomicAdd(&a.acc[SUR_GATED], n_gat);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (n_r[i]) atomicAdd(&a.acc[SUR_R_DOMAIN + i], n_r[i]);
+  }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -451,6 +761,21 @@ struct nrg_chem_handle {
   long long rates_cells = 0;
   long long chunk_max = 1ll << 23;
   size_t stage_cells = 0;
+  // learned step: features, network outputs, fallback list, counters
+  hipEvent_t ev2 = nullptr, ev3 = nullptr;
+  nrg::bf16 *sur_X = nullptr;
+  float *sur_Y = nullptr;
+  int32_t *sur_list = nullptr;
+  unsigned long long *sur_acc = nullptr;
+  long long sur_cells = 0;
+  // stiffness-ordered queue
+  int order = NRG_ORDER_NATURAL;
+  int32_t *order_list = nullptr;
+  int32_t *hist = nullptr;
+  unsigned long long *bins = nullptr;      // count[NBIN], cursor[NBIN]
+  long long order_cells = 0;
+  bool hist_valid = false;
+  GridIdx hist_grid{};
 };
 
 namespace {
@@ -665,6 +990,8 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     return set_error(NRG_CHEM_E_ARG, "jac_batch_pct must be in 0..100");
   if (o.waves_per_cu < 0 || o.waves_per_cu > 32)
     return set_error(NRG_CHEM_E_ARG, "waves_per_cu must be in 0..32");
+  if (o.cell_order != NRG_ORDER_NATURAL && o.cell_order != NRG_ORDER_STIFFNESS)
+    return set_error(NRG_CHEM_E_ARG, "unknown cell_order %d", o.cell_order);
   auto *h = new (std::nothrow) nrg_chem_handle();
   if (!h) return set_error(NRG_CHEM_E_ARG, "out of host memory");
   int st = build_mech(mech, h->host_mech);
@@ -677,6 +1004,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
   h->device = o.device;
   h->k = make_const(o);
   h->form = o.form;
+  h->order = o.cell_order;
   h->jac_pct = o.jac_batch_pct ? o.jac_batch_pct : 50;
   auto fail = [&](int s) {
     nrg_chem_destroy(h);
@@ -701,7 +1029,8 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
   }
   if (hipMemcpy(h->dev_mech, &h->host_mech, sizeof(MechCoef), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipMemcpy mech"));
-  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+      hipEventCreate(&h->ev2) != hipSuccess || hipEventCreate(&h->ev3) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipEventCreate"));
   *out = h;
   return NRG_CHEM_OK;
@@ -719,11 +1048,137 @@ void nrg_chem_destroy(nrg_chem_t *h)
   if (h->rates) hipFree(h->rates);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->ev2) hipEventDestroy(h->ev2);
+  if (h->ev3) hipEventDestroy(h->ev3);
+  for (void *p : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_acc,
+                  (void *)h->order_list, (void *)h->hist, (void *)h->bins})
+    if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
 
 void *nrg_chem_stream(nrg_chem_t *h) { return h ? (void *)h->stream : nullptr; }
+
+static bool same_grid(const GridIdx &x, const GridIdx &y)
+{
+  return std::memcmp(&x, &y, sizeof(GridIdx)) == 0;
+}
+
+// Build the stiffness-ordered cell list of this call into h->order_list and
+// point a.list / a.hist at it.  One small device->host copy (the bin counts).
+static int build_order(nrg_chem_t *h, StepArgs &a)
+{
+  const long long n = a.g.ninner;
+  if (n >= (1ll << 31)) return set_error(NRG_CHEM_E_ARG, "more than 2^31 inner cells");
+  if (n > h->order_cells) {
+    if (h->order_list) HIP_TRY(hipFree(h->order_list));
+    if (h->hist) HIP_TRY(hipFree(h->hist));
+    h->order_list = nullptr;
+    h->hist = nullptr;
+    h->order_cells = 0;
+    h->hist_valid = false;
+    HIP_TRY(hipMalloc(&h->order_list, (size_t)n * sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&h->hist, (size_t)n * sizeof(int32_t)));
+    h->order_cells = n;
+  }
+  if (!h->bins) HIP_TRY(hipMalloc(&h->bins, 2 * NBIN * sizeof(unsigned long long)));
+  BinArgs b{};
+  b.g = a.g;
+  b.T = a.T;
+  b.bc = a.bc;
+  b.t_gate = a.k.t_gate;
+  b.hist = (h->hist_valid && same_grid(h->hist_grid, a.g)) ? h->hist : nullptr;
+  b.count = h->bins;
+  b.cursor = h->bins + NBIN;
+  b.list = h->order_list;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(2 * NBIN), 0, h->stream, h->bins, 2 * NBIN);
+  hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, h->stream, b);
+  unsigned long long cnt[NBIN], cur[NBIN];
+  HIP_TRY(hipMemcpyAsync(cnt, h->bins, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  unsigned long long pos = 0;
+  for (int k = NBIN - 1; k >= 0; --k) {
+    cur[k] = pos;
+    pos += cnt[k];
+  }
+  if ((long long)pos != n) return set_error(NRG_CHEM_E_HIP, "cell binning counted %llu of %lld cells", pos, n);
+  HIP_TRY(hipMemcpyAsync(h->bins + NBIN, cur, sizeof(cur), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(nb), dim3(256), 0, h->stream, b);
+  HIP_TRY(hipGetLastError());
+  // the copy from the host array must complete before `cur` goes out of scope
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  a.list = h->order_list;
+  a.hist = h->hist;
+  h->hist_valid = true;
+  h->hist_grid = a.g;
+  return NRG_CHEM_OK;
+}
+
+// Enqueue rates_kernel + chem_bdf_kernel over npos queue positions (all inner
+// cells when a.list is NULL, else the cells a.list names), in chunks of at
+// most chunk_max positions, each with its own queue head.
+static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
+{
+  long long chunk = std::min<long long>(npos, h->chunk_max);
+  const long long min_chunk = (npos + MAX_CHUNKS - 1) / MAX_CHUNKS;
+  chunk = std::max<long long>(std::max<long long>(chunk, min_chunk), 1);
+  const long long nchunks = npos > 0 ? (npos + chunk - 1) / chunk : 0;
+  if (npos > 0 && h->rates_cells < chunk) {
+    if (h->rates) HIP_TRY(hipFree(h->rates));
+    h->rates = nullptr;
+    h->rates_cells = 0;
+    HIP_TRY(hipMalloc(&h->rates, (size_t)chunk * (2 * R + NM) * sizeof(double)));
+    h->rates_cells = chunk;
+  }
+  a.rates = h->rates;
+  a.rates_out = h->rates;
+  for (long long ci = 0; ci < nchunks; ++ci) {
+    a.base = ci * chunk;
+    a.nchunk = std::min<long long>(chunk, npos - a.base);
+    a.queue = h->dev_acc + ACC_QUEUE0 + ci;
+    hipLaunchKernelGGL(rates_kernel, dim3((unsigned)((a.nchunk + 255) / 256)), dim3(256), 0,
+                       h->stream, a);
+    const long long need = (a.nchunk + WAVE - 1) / WAVE;
+    if (h->form == NRG_FORM_PERSISTENT) {
+      const long long nb = std::min<long long>(h->persistent_blocks, need);
+      hipLaunchKernelGGL(chem_bdf_kernel<true>, dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
+    } else {
+      hipLaunchKernelGGL(chem_bdf_kernel<false>, dim3((unsigned)need), dim3(WAVE), 0, h->stream, a);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return NRG_CHEM_OK;
+}
+
+static void fill_stats(const unsigned long long (&acc)[16], long long ninner, float ms,
+                       nrg_chem_stats *stats)
+{
+  std::memset(stats, 0, sizeof(*stats));
+  stats->cells_inner = ninner;
+  stats->cells_integrated = (int64_t)acc[ACC_INTEG];
+  stats->nstep_sum = (int64_t)acc[ACC_NSTEP];
+  stats->nfe_sum = (int64_t)acc[ACC_NFE];
+  stats->nje_sum = (int64_t)acc[ACC_NJE];
+  stats->ncorr_sum = (int64_t)acc[ACC_NCORR];
+  stats->natt_sum = (int64_t)acc[ACC_NATT];
+  stats->nstep_max = (int32_t)acc[ACC_NSTEP_MAX];
+  stats->nfe_max = (int32_t)acc[ACC_NFE_MAX];
+  for (int i = 0; i < 6; ++i) stats->status_count[i] = (int64_t)acc[ACC_STATUS0 + i];
+  stats->kernel_ms = ms;
+}
+
+// The reference STOPs on these (slatec/ddriv3.f90 IERFLG 41, 42); here they are statuses.
+static int integrator_status(const unsigned long long (&acc)[16])
+{
+  if (acc[ACC_STATUS0 + NRG_CELL_STEP_ZERO])
+    return set_error(NRG_CHEM_E_STEP_ZERO, "%llu cell(s): step size went to zero",
+                     acc[ACC_STATUS0 + NRG_CELL_STEP_ZERO]);
+  if (acc[ACC_STATUS0 + NRG_CELL_REPEATED_FAIL] || acc[ACC_STATUS0 + NRG_CELL_SINGULAR])
+    return set_error(NRG_CHEM_E_REPEATED_FAIL, "%llu cell(s): repeated integrator failures",
+                     acc[ACC_STATUS0 + NRG_CELL_REPEATED_FAIL] + acc[ACC_STATUS0 + NRG_CELL_SINGULAR]);
+  return NRG_CHEM_OK;
+}
 
 static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
                          const double *rho, const double *const *Y, const int8_t *bc,
@@ -758,68 +1213,21 @@ static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const d
   a.acc = h->dev_acc;
   a.jac_pct = h->jac_pct;
   HIP_TRY(hipSetDevice(h->device));
-  // cells per chunk: at most chunk_max, and few enough chunks for the queue slots
-  long long chunk = std::min<long long>(a.g.ninner, h->chunk_max);
-  const long long min_chunk = (a.g.ninner + MAX_CHUNKS - 1) / MAX_CHUNKS;
-  chunk = std::max<long long>(std::max<long long>(chunk, min_chunk), 1);
-  const long long nchunks = a.g.ninner > 0 ? (a.g.ninner + chunk - 1) / chunk : 0;
-  if (a.g.ninner > 0 && h->rates_cells < chunk) {
-    if (h->rates) HIP_TRY(hipFree(h->rates));
-    h->rates = nullptr;
-    h->rates_cells = 0;
-    HIP_TRY(hipMalloc(&h->rates, (size_t)chunk * (2 * R + NM) * sizeof(double)));
-    h->rates_cells = chunk;
-  }
-  a.rates = h->rates;
-  a.rates_out = h->rates;
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
   HIP_TRY(hipGetLastError());
   unsigned long long acc[16] = {};
   float ms = 0.f;
   if (a.g.ninner > 0) {
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    for (long long ci = 0; ci < nchunks; ++ci) {
-      a.base = ci * chunk;
-      a.nchunk = std::min<long long>(chunk, a.g.ninner - a.base);
-      a.queue = h->dev_acc + ACC_QUEUE0 + ci;
-      hipLaunchKernelGGL(rates_kernel, dim3((unsigned)((a.nchunk + 255) / 256)), dim3(256), 0,
-                         h->stream, a);
-      const long long need = (a.nchunk + WAVE - 1) / WAVE;
-      if (h->form == NRG_FORM_PERSISTENT) {
-        const long long nb = std::min<long long>(h->persistent_blocks, need);
-        hipLaunchKernelGGL(chem_bdf_kernel<true>, dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
-      } else {
-        hipLaunchKernelGGL(chem_bdf_kernel<false>, dim3((unsigned)need), dim3(WAVE), 0, h->stream,
-                           a);
-      }
-    }
-    HIP_TRY(hipGetLastError());
+    if (h->order == NRG_ORDER_STIFFNESS && (st = build_order(h, a))) return st;
+    if ((st = enqueue_integrator(h, a, a.g.ninner))) return st;
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipMemcpyAsync(acc, h->dev_acc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   }
-  if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
-    stats->cells_inner = a.g.ninner;
-    stats->cells_integrated = (int64_t)acc[ACC_INTEG];
-    stats->nstep_sum = (int64_t)acc[ACC_NSTEP];
-    stats->nfe_sum = (int64_t)acc[ACC_NFE];
-    stats->nje_sum = (int64_t)acc[ACC_NJE];
-    stats->ncorr_sum = (int64_t)acc[ACC_NCORR];
-    stats->natt_sum = (int64_t)acc[ACC_NATT];
-    stats->nstep_max = (int32_t)acc[ACC_NSTEP_MAX];
-    stats->nfe_max = (int32_t)acc[ACC_NFE_MAX];
-    for (int i = 0; i < 6; ++i) stats->status_count[i] = (int64_t)acc[ACC_STATUS0 + i];
-    stats->kernel_ms = ms;
-  }
-  if (acc[ACC_STATUS0 + NRG_CELL_STEP_ZERO])
-    return set_error(NRG_CHEM_E_STEP_ZERO, "%llu cell(s): step size went to zero",
-                     acc[ACC_STATUS0 + NRG_CELL_STEP_ZERO]);
-  if (acc[ACC_STATUS0 + NRG_CELL_REPEATED_FAIL] || acc[ACC_STATUS0 + NRG_CELL_SINGULAR])
-    return set_error(NRG_CHEM_E_REPEATED_FAIL, "%llu cell(s): repeated integrator failures",
-                     acc[ACC_STATUS0 + NRG_CELL_REPEATED_FAIL] + acc[ACC_STATUS0 + NRG_CELL_SINGULAR]);
-  return NRG_CHEM_OK;
+  if (stats) fill_stats(acc, a.g.ninner, ms, stats);
+  return integrator_status(acc);
 }
 
 int nrg_chem_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
@@ -990,6 +1398,129 @@ int nrg_energy_from_temperature_dev(nrg_chem_t *h, const nrg_grid *grid, const d
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
   return NRG_CHEM_OK;
+}
+
+int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surrogate_norm *norm,
+                                const nrg_surrogate_gate *gate, const nrg_grid *grid, double dt,
+                                const double *T, const double *p, const double *rho,
+                                const double *const *Y, const int8_t *bc,
+                                double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                                int32_t *cell_status, nrg_surrogate_stats *stats)
+{
+  if (!h || !mlp || !norm || !gate) return set_error(NRG_CHEM_E_ARG, "handle, network, norm or gate is NULL");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return set_error(NRG_CHEM_E_ARG, "time_step must be > 0");
+  if (!T || !p || !rho || !Y || !Y_prod || !Y_prod2 || !E_f_prod)
+    return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL (the network needs p)");
+  if (mlp_n_in(mlp) != NIN || mlp_n_out(mlp) != NOUT)
+    return set_error(NRG_CHEM_E_ARG, "network maps %d -> %d, the step needs %d -> %d", mlp_n_in(mlp),
+                     mlp_n_out(mlp), NIN, NOUT);
+  if (gate->n_elements < 0 || gate->n_elements > NRG_SUR_MAX_ELEMENTS)
+    return set_error(NRG_CHEM_E_ARG, "n_elements must be in 0..%d", NRG_SUR_MAX_ELEMENTS);
+  StepArgs a{};
+  int st = make_grid(grid, a.g);
+  if (st) return st;
+  if (a.g.ninner >= (1ll << 31)) return set_error(NRG_CHEM_E_ARG, "more than 2^31 inner cells");
+  SurArgs sa{};
+  sa.g = a.g;
+  for (int s = 0; s < S; ++s) {
+    if (!Y[s] || !Y_prod[s] || !Y_prod2[s]) return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", s + 1);
+    a.Y[s] = sa.Y[s] = Y[s];
+    a.Yp[s] = sa.Yp[s] = Y_prod[s];
+    a.Yp2[s] = sa.Yp2[s] = Y_prod2[s];
+  }
+  a.dt = sa.dt = dt;
+  a.k = h->k;
+  a.mech = sa.mech = h->dev_mech;
+  a.T = sa.T = T;
+  a.rho = sa.rho = rho;
+  sa.p = p;
+  a.bc = sa.bc = bc;
+  a.Efp = sa.Efp = E_f_prod;
+  a.status = sa.status = cell_status;
+  a.acc = h->dev_acc;
+  a.jac_pct = h->jac_pct;
+  sa.t_gate = h->k.t_gate;
+  for (int i = 0; i < NIN; ++i) {
+    sa.in_shift[i] = norm->in_shift[i];
+    sa.in_scale[i] = norm->in_scale[i];
+  }
+  for (int i = 0; i < NOUT; ++i) {
+    sa.out_shift[i] = norm->out_shift[i];
+    sa.out_scale[i] = norm->out_scale[i];
+  }
+  sa.y_floor = norm->y_floor;
+  sa.t_min = gate->t_min; sa.t_max = gate->t_max;
+  sa.p_min = gate->p_min; sa.p_max = gate->p_max;
+  sa.dt_ok = dt >= gate->dt_min && dt <= gate->dt_max;
+  sa.elem_tol = gate->elem_tol; sa.elem_floor = gate->elem_floor;
+  sa.y_neg_tol = gate->y_neg_tol; sa.energy_tol = gate->energy_tol;
+  sa.ne = gate->n_elements;
+  for (int e = 0; e < sa.ne; ++e)
+    for (int s = 0; s < S; ++s) sa.E[e][s] = gate->elements[e][s];
+
+  HIP_TRY(hipSetDevice(h->device));
+  const long long n = a.g.ninner;
+  if (n > h->sur_cells) {
+    for (void *q : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list})
+      if (q) HIP_TRY(hipFree(q));
+    h->sur_X = nullptr; h->sur_Y = nullptr; h->sur_list = nullptr; h->sur_cells = 0;
+    HIP_TRY(hipMalloc(&h->sur_X, (size_t)n * MLP_IN_PAD * sizeof(bf16)));
+    HIP_TRY(hipMalloc(&h->sur_Y, (size_t)n * NOUT * sizeof(float)));
+    HIP_TRY(hipMalloc(&h->sur_list, (size_t)n * sizeof(int32_t)));
+    h->sur_cells = n;
+  }
+  if (!h->sur_acc) HIP_TRY(hipMalloc(&h->sur_acc, SUR_SLOTS * sizeof(unsigned long long)));
+  sa.X = h->sur_X;
+  sa.Ypred = h->sur_Y;
+  sa.list = h->sur_list;
+  sa.acc = h->sur_acc;
+
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(SUR_SLOTS), 0, h->stream, h->sur_acc, SUR_SLOTS);
+  HIP_TRY(hipGetLastError());
+  unsigned long long sur[SUR_SLOTS] = {}, acc[16] = {};
+  float ms_inf = 0.f, ms_gate = 0.f, ms_fb = 0.f;
+  if (n > 0) {
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    hipLaunchKernelGGL(featurize_kernel, dim3(nb), dim3(256), 0, h->stream, sa);
+    HIP_TRY(hipGetLastError());
+    if ((st = mlp_infer_bf16(mlp, n, h->sur_X, h->sur_Y, h->stream))) return st;
+    HIP_TRY(hipEventRecord(h->ev2, h->stream));
+    hipLaunchKernelGGL(gate_kernel, dim3(nb), dim3(256), 0, h->stream, sa);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->ev3, h->stream));
+    HIP_TRY(hipMemcpyAsync(sur, h->sur_acc, sizeof(sur), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const long long nfb = (long long)sur[SUR_FALLBACK];
+    if (nfb > n) return set_error(NRG_CHEM_E_HIP, "fallback count %lld exceeds %lld cells", nfb, n);
+    if (nfb > 0) {
+      a.list = h->sur_list;
+      if ((st = enqueue_integrator(h, a, nfb))) return st;
+    }
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipMemcpyAsync(acc, h->dev_acc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipEventElapsedTime(&ms_inf, h->ev0, h->ev2));
+    HIP_TRY(hipEventElapsedTime(&ms_gate, h->ev2, h->ev3));
+    HIP_TRY(hipEventElapsedTime(&ms_fb, h->ev3, h->ev1));
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->cells_inner = n;
+    stats->cells_gated = (int64_t)sur[SUR_GATED];
+    stats->cells_accepted = (int64_t)sur[SUR_ACCEPT];
+    stats->cells_fallback = (int64_t)sur[SUR_FALLBACK];
+    stats->reject_domain = (int64_t)sur[SUR_R_DOMAIN];
+    stats->reject_elements = (int64_t)sur[SUR_R_ELEM];
+    stats->reject_negative = (int64_t)sur[SUR_R_NEG];
+    stats->reject_energy = (int64_t)sur[SUR_R_ENERGY];
+    stats->infer_ms = ms_inf;
+    stats->gate_ms = ms_gate;
+    stats->fallback_ms = ms_fb;
+    fill_stats(acc, (long long)sur[SUR_FALLBACK], ms_fb, &stats->fallback);
+  }
+  return integrator_status(acc);
 }
 
 }  // extern "C"

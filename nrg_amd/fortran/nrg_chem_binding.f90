@@ -25,6 +25,7 @@ module nrg_chem_binding
                                        NRG_CELL_SINGULAR = 5
   integer(c_int), parameter, public :: NRG_JAC_FD_SLATEC = 0, NRG_JAC_ANALYTIC = 1
   integer(c_int), parameter, public :: NRG_FORM_PLAIN = 0, NRG_FORM_PERSISTENT = 1
+  integer(c_int), parameter, public :: NRG_ORDER_NATURAL = 0, NRG_ORDER_STIFFNESS = 1
 
   ! struct nrg_chem_mech: the arrays of chemical_properties / thermophysical_properties
   type, bind(C), public :: nrg_chem_mech
@@ -37,7 +38,8 @@ module nrg_chem_binding
   type, bind(C), public :: nrg_chem_opts
     real(c_double) :: eps, ewt, hmax, t_gate
     integer(c_int32_t) :: mxstep, mxord, jacobian, device, form, jac_batch_pct, waves_per_cu
-    integer(c_int32_t) :: reserved(5)
+    integer(c_int32_t) :: cell_order
+    integer(c_int32_t) :: reserved(4)
   end type
 
   type, bind(C), public :: nrg_grid

@@ -166,3 +166,20 @@ def ignition_mix(mech, n, seed=77, device="cpu", frac_window=0.05):
 
 def to_numpy(state):
     return {k: v.detach().cpu().numpy() for k, v in state.items()}
+
+
+def training_states(mech, n, seed=11, device="cpu", T_range=(300.0, 2800.0), phi_range=(0.3, 3.0),
+                    p_atm_range=(0.5, 5.0), burnt_fraction=0.5):
+    """Initial states of the surrogate's training trajectories: T ~ U[T_range],
+    phi log-uniform in phi_range, p ~ U[p_atm_range] atm; half fresh mixture
+    (progress 0), half blended towards burnt gas with progress ~ U[0, 1]."""
+    torch = _torch()
+    W = torch.tensor(mech.molar_masses, dtype=torch.float64, device=device)
+    g = _gen(seed, device)
+    u = lambda: torch.rand(n, generator=g, dtype=torch.float64, device=device)
+    T = T_range[0] + (T_range[1] - T_range[0]) * u()
+    phi = torch.exp(math.log(phi_range[0]) + (math.log(phi_range[1]) - math.log(phi_range[0])) * u())
+    p = P_ATM * (p_atm_range[0] + (p_atm_range[1] - p_atm_range[0]) * u())
+    c = torch.where(u() < burnt_fraction, u(), torch.zeros(n, dtype=torch.float64, device=device))
+    X = _compose(phi, c, W)
+    return _finish(T, p, X, W)

@@ -1,6 +1,6 @@
-"""The C-ABI library loads and exports exactly what include/nrg_chem.h declares,
-and the Python bindings agree with the declared signatures.  No GPU needed:
-nothing here launches device work."""
+"""The C-ABI library loads and exports every entry point that include/*.h
+declares, and the Python bindings agree with the declared signatures.  No GPU
+needed: nothing here launches device work."""
 import ctypes as C
 import os
 import re
@@ -10,14 +10,16 @@ import pytest
 from nrg_amd import capi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "nrg_chem.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
 
 
 def _declarations():
-    txt = open(HEADER).read()
+    txt = "".join(open(h).read() for h in HEADERS)
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     decls = {}
-    for m in re.finditer(r"\b(?:int|void|const char)\s*\*?\s*(nrg_\w+)\s*\(([^;]*?)\)\s*;", txt):
+    for m in re.finditer(r"\b(?:int|void|const char|int64_t)\s*\*?\s*(nrg_\w+)\s*\(([^;]*?)\)\s*;",
+                         txt):
         name, params = m.group(1), m.group(2).strip()
         n = 0 if params in ("", "void") else params.count(",") + 1
         decls[name] = n
@@ -29,7 +31,10 @@ def test_header_declares_entry_points():
     for name in ("nrg_chem_create", "nrg_chem_destroy", "nrg_chem_step", "nrg_chem_step_dev",
                  "nrg_chem_step_dev_diag", "nrg_chem_closure_dev", "nrg_state_equation_dev",
                  "nrg_energy_from_temperature_dev", "nrg_chem_default_opts", "nrg_chem_strerror",
-                 "nrg_chem_last_error", "nrg_chem_abi_version", "nrg_chem_stream"):
+                 "nrg_chem_last_error", "nrg_chem_abi_version", "nrg_chem_stream",
+                 "nrg_mlp_create", "nrg_mlp_destroy", "nrg_mlp_num_params", "nrg_mlp_set_params",
+                 "nrg_mlp_get_params", "nrg_mlp_forward_dev", "nrg_mlp_grad_dev",
+                 "nrg_mlp_train_step_dev", "nrg_mlp_stream", "nrg_chem_surrogate_step_dev"):
         assert name in d, name
 
 
@@ -54,6 +59,9 @@ def test_struct_layouts():
     assert C.sizeof(capi.NrgChemOpts) == 4 * 8 + 12 * 4
     assert C.sizeof(capi.NrgGrid) == 12 * 4
     assert C.sizeof(capi.NrgChemStats) == 7 * 8 + 2 * 4 + 8 * 8 + 8
+    assert C.sizeof(capi.NrgSurrogateNorm) == 4 * 16 * 4 + 4 + 7 * 4
+    assert C.sizeof(capi.NrgSurrogateGate) == 10 * 8 + 4 + 7 * 4 + 4 * 32 * 8
+    assert C.sizeof(capi.NrgSurrogateStats) == 8 * 8 + 3 * 8 + C.sizeof(capi.NrgChemStats)
 
 
 def test_default_opts_are_reference_literals():

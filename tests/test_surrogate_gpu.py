@@ -1,0 +1,179 @@
+"""The learned step's gate and FP64 fallback (include/nrg_surrogate.h) on
+networks whose answer is known exactly: with all weights zero the network
+returns out_shift, so every gate decision and every projected increment can
+be checked, and rejected cells must reproduce nrg_chem_step_dev bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from nrg_amd import capi, inputs
+from nrg_amd.mechanism import element_matrix
+
+pytestmark = pytest.mark.gpu
+S = 9
+
+
+def _report(name, **kw):
+    print("[surrogate] %s " % name + " ".join("%s=%s" % (k, v) for k, v in kw.items()), flush=True)
+
+
+@pytest.fixture(scope="module")
+def zero_net():
+    from nrg_amd.surrogate import MLP
+    net = MLP(S + 3, S + 1)
+    net.set_params(np.zeros(net.num_params, dtype=np.float32))
+    yield net
+    net.close()
+
+
+def _norm(shift):
+    from nrg_amd.surrogate import Normalizer
+    return Normalizer(np.zeros(S + 3), np.ones(S + 3), shift, np.ones(S + 1))
+
+
+def _gate(mech, **kw):
+    from nrg_amd.surrogate import gate_struct
+    base = dict(t_range=(0.0, 1e4), p_range=(0.0, 1e9), dt_range=(1e-12, 1.0), elem_tol=1e9,
+                elem_floor=1e9, y_neg_tol=1.0, energy_tol=1e9)
+    base.update(kw)
+    return gate_struct(mech, **base)
+
+
+def _state(mech, n=3000, seed=5):
+    import torch
+    st = inputs.ignition_mix(mech, n, seed=seed, frac_window=0.3, device="cuda:0")
+    st["T"][:7] = 250.0                     # a few cold (gated) cells
+    bc = torch.zeros(n, dtype=torch.int8, device="cuda:0")
+    bc[7:12] = 1                            # and boundary cells
+    return st, bc
+
+
+def _run(engine, net, norm, gate, st, bc, dt):
+    import torch
+    n = st["T"].shape[0]
+    Yp = torch.zeros((S, n), dtype=torch.float64, device="cuda:0")
+    Yp2 = torch.full((S, n), 7.0, dtype=torch.float64, device="cuda:0")
+    Ef = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    status = torch.full((n,), -1, dtype=torch.int32, device="cuda:0")
+    Y = st["Y"].contiguous()
+    stats = capi.NrgSurrogateStats()
+    torch.cuda.synchronize()
+    rc = engine.lib.nrg_chem_surrogate_step_dev(
+        engine.h, net.h, C.byref(norm.struct()), C.byref(gate), C.byref(capi.flat_grid(n)), dt,
+        st["T"].data_ptr(), st["p"].data_ptr(), st["rho"].data_ptr(),
+        capi.ptr_array([Y[s].data_ptr() for s in range(S)]), bc.data_ptr(),
+        capi.ptr_array([Yp[s].data_ptr() for s in range(S)]),
+        capi.ptr_array([Yp2[s].data_ptr() for s in range(S)]), Ef.data_ptr(), status.data_ptr(),
+        C.byref(stats))
+    capi.check(engine.lib, rc)
+    return dict(Y_prod=Yp.cpu().numpy(), Y_prod2=Yp2.cpu().numpy(), E_f_prod=Ef.cpu().numpy(),
+                status=status.cpu().numpy(), stats=stats)
+
+
+def _fp64(engine, st, bc, dt):
+    import torch
+    n = st["T"].shape[0]
+    Yp = torch.zeros((S, n), dtype=torch.float64, device="cuda:0")
+    Yp2 = torch.full((S, n), 7.0, dtype=torch.float64, device="cuda:0")
+    Ef = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    status = torch.full((n,), -1, dtype=torch.int32, device="cuda:0")
+    Y = st["Y"].contiguous()
+    engine.step_dev(capi.flat_grid(n), dt, st["T"].data_ptr(), st["rho"].data_ptr(),
+                    [Y[s].data_ptr() for s in range(S)], [Yp[s].data_ptr() for s in range(S)],
+                    [Yp2[s].data_ptr() for s in range(S)], Ef.data_ptr(), bc=bc.data_ptr(),
+                    status=status.data_ptr())
+    torch.cuda.synchronize()
+    return dict(Y_prod=Yp.cpu().numpy(), Y_prod2=Yp2.cpu().numpy(), E_f_prod=Ef.cpu().numpy(),
+                status=status.cpu().numpy())
+
+
+def test_accept_all_zero_increment(engine, mech, zero_net):
+    st, bc = _state(mech)
+    o = _run(engine, zero_net, _norm(np.zeros(S + 1)), _gate(mech), st, bc, 1e-7)
+    T = st["T"].cpu().numpy()
+    gated = (T < 305.0) | (bc.cpu().numpy() != 0)
+    s = o["stats"]
+    _report("accept_all", accepted=s.cells_accepted, gated=s.cells_gated, fallback=s.cells_fallback)
+    assert s.cells_gated == gated.sum() and s.cells_accepted == (~gated).sum()
+    assert s.cells_fallback == 0
+    assert np.all(o["status"][~gated] == capi.NRG_CELL_SURROGATE)
+    assert np.all(o["status"][gated] == capi.NRG_CELL_GATED)
+    assert np.all(o["Y_prod"] == 0) and np.all(o["E_f_prod"] == 0)
+    assert np.all(o["Y_prod2"][:, ~gated] == 0) and np.all(o["Y_prod2"][:, gated] == 7.0)
+
+
+def test_reject_all_is_the_fp64_step(engine, mech, zero_net):
+    st, bc = _state(mech, seed=8)
+    o = _run(engine, zero_net, _norm(np.zeros(S + 1)), _gate(mech, energy_tol=-1.0), st, bc, 1e-7)
+    f = _fp64(engine, st, bc, 1e-7)
+    s = o["stats"]
+    _report("reject_all", fallback=s.cells_fallback, reject_energy=s.reject_energy,
+            nfe=s.fallback.nfe_sum)
+    assert s.cells_accepted == 0 and s.cells_fallback == s.reject_energy > 0
+    for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+        assert np.array_equal(o[k], f[k]), k
+
+
+def test_domain_split(engine, mech, zero_net):
+    st, bc = _state(mech, seed=9)
+    o = _run(engine, zero_net, _norm(np.zeros(S + 1)), _gate(mech, t_range=(1500.0, 1e4)), st, bc,
+             1e-7)
+    f = _fp64(engine, st, bc, 1e-7)
+    T = st["T"].cpu().numpy()
+    gated = (T < 305.0) | (bc.cpu().numpy() != 0)
+    rej = ~gated & (T < 1500.0)
+    acc = ~gated & (T >= 1500.0)
+    s = o["stats"]
+    assert s.reject_domain == rej.sum() and s.cells_accepted == acc.sum()
+    assert np.array_equal(o["Y_prod"][:, rej], f["Y_prod"][:, rej])
+    assert np.array_equal(o["status"][rej], f["status"][rej])
+    assert np.all(o["Y_prod"][:, acc] == 0)
+
+
+def _reaction_increment(mech, a, off=1e-3):
+    """dY of a mol/kg of H2 + 1/2 O2 -> H2O, with H2O overshooting by `off` (not conserving)."""
+    W = mech.molar_masses
+    d = np.zeros(S + 1)
+    d[1 + 0] = -a * W[0]
+    d[1 + 1] = -0.5 * a * W[1]
+    d[1 + 6] = a * W[6] * (1.0 + off)
+    return d
+
+
+def test_projection_conserves_elements(engine, mech, zero_net):
+    st, bc = _state(mech, seed=10)
+    shift = _reaction_increment(mech, 1e-3)
+    o = _run(engine, zero_net, _norm(shift), _gate(mech), st, bc, 1e-7)
+    acc = o["status"] == capi.NRG_CELL_SURROGATE
+    assert acc.sum() > 100
+    E = element_matrix(mech.species_names)
+    dC = o["Y_prod2"][:, acc]                      # mol/m3
+    res = np.abs(E @ dC)
+    scale = np.abs(E) @ np.abs(dC)
+    _report("projection", max_rel_residual=float((res / scale).max()))
+    assert np.all(res <= 1e-12 * scale)
+    # the correction is small: the raw increment is off by 1e-3 in H2O only
+    rho = st["rho"].cpu().numpy()[acc]
+    dY = o["Y_prod"][:, acc] * 1e-7 / rho
+    raw = shift[1:][:, None]
+    assert np.all(np.abs(dY - raw) <= 2e-3 * np.abs(raw).max())
+    # energy released by the accepted chemistry: E_f_prod = -sum h_s(T_ref) dC_s / dt > 0
+    assert np.all(o["E_f_prod"][acc] > 0)
+
+
+def test_element_and_energy_gates_reject(engine, mech, zero_net):
+    st, bc = _state(mech, seed=12)
+    shift = _reaction_increment(mech, 1e-3, off=0.2)     # 20 % H2O overshoot
+    o = _run(engine, zero_net, _norm(shift), _gate(mech, elem_tol=0.05, elem_floor=0.0), st, bc,
+             1e-7)
+    s = o["stats"]
+    live = s.cells_inner - s.cells_gated
+    assert s.reject_elements == live and s.cells_accepted == 0
+    # conserving increment but a network dT of 0 while heat is released
+    o = _run(engine, zero_net, _norm(_reaction_increment(mech, 1e-3, off=0.0)),
+             _gate(mech, energy_tol=1e-3), st, bc, 1e-7)
+    s = o["stats"]
+    _report("gates", reject_energy=s.reject_energy, live=live)
+    assert s.reject_energy == live
