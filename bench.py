@@ -43,6 +43,16 @@ def parse():
                     help="queue order of the cells (same results)")
     ap.add_argument("--jac-pct", type=int, default=0,
                     help="batched-Jacobian threshold in percent of pending lanes (0: default)")
+    ap.add_argument("--surrogate", default="auto",
+                    help="trained surrogate (.npz) timed beside the FP64 integrator from the same "
+                         "initial state; 'auto': nrg_amd/data/surrogate_keromnes.npz if present; "
+                         "'none': skip")
+    ap.add_argument("--gate-energy-tol", type=float, default=None, help="K (default: trained file's)")
+    ap.add_argument("--gate-elem-tol", type=float, default=None)
+    ap.add_argument("--gate-elem-floor", type=float, default=None, help="mol/kg")
+    ap.add_argument("--gate-dt-max", type=float, default=None, help="K")
+    ap.add_argument("--gate-dt-freeze", type=float, default=None, help="K")
+    ap.add_argument("--gate-dy-freeze", type=float, default=None)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this size before the timed region, as the "
                          "known-size reference for HBM counters (FETCH_SIZE / WRITE_SIZE)")
@@ -99,6 +109,10 @@ def main():
                       Y=st["Y"].cpu().numpy()[:, idx],
                       E_f=E_f.inner(g).reshape(-1).cpu().numpy()[idx])
     del st
+    # initial state, kept for the surrogate leg (which starts from the same fields)
+    fields0 = {k: dm.fields[k].data.clone() for k in ("temperature", "pressure", "density",
+                                                      "full_energy")}
+    Y0 = [Yf.pr[s].data.clone() for s in range(S)]
     torch.cuda.synchronize()
 
     def split_step():
@@ -188,6 +202,15 @@ def main():
             audit = {"cells": m, "max_abs_Y": float(dY), "max_rel_T": relT,
                      "bar_Y": 1e-9, "bar_T": 1e-10}
 
+    # ---- surrogate + FP64 fallback from the same initial state ---------------------------
+    sur = None
+    path = args.surrogate
+    if path == "auto":
+        path = os.path.join(ROOT, "nrg_amd", "data", "surrogate_keromnes.npz")
+        path = path if os.path.exists(path) else "none"
+    if path != "none":
+        sur = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, value, S)
+
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -212,8 +235,109 @@ def main():
         "audit_vs_oracle": audit,
         "work": {"nfe_per_step": nfe / args.steps, "integrated_cells_per_step": integ / args.steps,
                  "nstep_max": nstep_max},
+        "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
+
+
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, fp64_value, S):
+    """Time the learned split step (network + gate + FP64 fallback + closure)
+    over the same number of steps from the same initial state, and audit its
+    final state against the FP64 integrator's."""
+    import numpy as np
+    import torch
+
+    from nrg_amd import surrogate as sg
+    g = dm.grid
+    Yf = dm.fields["specie_mass_fraction"]
+    # the FP64 run's final state
+    T_ref = dm.fields["temperature"].inner(g).clone()
+    Y_ref = [Yf.pr[s].inner(g).clone() for s in range(S)]
+    # back to the initial state
+    for k, v in fields0.items():
+        dm.fields[k].data.copy_(v)
+    for s in range(S):
+        Yf.pr[s].data.copy_(Y0[s])
+    z = np.load(path, allow_pickle=False)
+    mlp, norm = sg.load(path)
+    gkw = {}
+    if args.gate_energy_tol is not None:
+        gkw["energy_tol"] = args.gate_energy_tol
+    if args.gate_elem_tol is not None:
+        gkw["elem_tol"] = args.gate_elem_tol
+    if args.gate_elem_floor is not None:
+        gkw["elem_floor"] = args.gate_elem_floor
+    for k, v in (("dT_max", args.gate_dt_max), ("dT_freeze", args.gate_dt_freeze),
+                 ("dY_freeze", args.gate_dy_freeze)):
+        if v is not None:
+            gkw[k] = v
+    for k in ("dT_max", "dT_freeze", "dY_freeze", "energy_tol", "elem_tol", "elem_floor"):
+        if k not in gkw and ("gate_" + k) in z.files:
+            gkw[k] = float(z["gate_" + k])
+    gate = sg.gate_struct(mech, **gkw)
+    chem = sg.SurrogateChemistry(solver, mlp, norm, gate)
+
+    def step():
+        st = chem.solve_chemical_kinetics(args.dt)
+        chem.apply_split_sources(args.dt)
+        return st
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    recs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        recs.append(step())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ncells = dm.inner_cells()
+    value = ncells * args.steps / wall
+    mean = lambda f: float(np.mean([f(r) for r in recs]))
+    infer_ms = mean(lambda r: r.infer_ms)
+    flops = 2 * (12 * 256 + 3 * 256 * 256 + 256 * 10) * ncells      # 404,480 per inferred cell
+    tflops = flops / (infer_ms * 1e-3) / 1e12
+    # audit: final state against the FP64 run's final state (all inner cells)
+    T_s = dm.fields["temperature"].inner(g)
+    relT = ((T_s - T_ref).abs() / T_ref).reshape(-1)
+    major = (0, 1, 6)       # H2, O2, H2O
+    dY = torch.stack([(Yf.pr[s].inner(g) - Y_ref[s]).abs().reshape(-1) for s in major])
+    dY_all = torch.stack([(Yf.pr[s].inner(g) - Y_ref[s]).abs().reshape(-1) for s in range(S)])
+    q = lambda t, p: float(torch.quantile(t[::max(1, t.numel() // 4_000_000)].float(), p))
+    res = {
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "speedup_vs_fp64": round(value / fp64_value, 3),
+        "dtype": "bf16 (fp32 accumulate) network, f64 gate and fallback",
+        "fallback_frac": round(mean(lambda r: r.cells_fallback) / ncells, 6),
+        "accepted_frac": round(mean(lambda r: r.cells_accepted) / ncells, 6),
+        "frozen_frac": round(mean(lambda r: r.cells_frozen) / ncells, 6),
+        "fallback_nfe_per_step": mean(lambda r: r.fallback.nfe_sum),
+        "fallback_nstep_max": max(r.fallback.nstep_max for r in recs),
+        "rejects_per_step": {k: mean(lambda r, k=k: getattr(r, k)) for k in
+                             ("reject_domain", "reject_confidence", "reject_elements",
+                              "reject_negative", "reject_energy")},
+        "gate": gkw,
+        "infer_ms": round(infer_ms, 3), "gate_ms": round(mean(lambda r: r.gate_ms), 3),
+        "fallback_ms": round(mean(lambda r: r.fallback_ms), 3),
+        "roofline_infer": {"bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_BF16_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 4),
+                           "kernel": "featurize_kernel + mlp_infer_kernel (404,480 FLOP/cell)"},
+        "audit_vs_fp64": {
+            "steps": args.warmup + args.steps, "cells": ncells,
+            "max_rel_T": float(relT.max()), "p99_rel_T": q(relT, 0.99),
+            "max_abs_Y_major": float(dY.max()), "p99_abs_Y_major": q(dY.max(dim=0).values, 0.99),
+            "max_abs_Y_any": float(dY_all.max()),
+            "bar_rel_T": 1e-2, "bar_abs_Y_major": 1e-3,
+        },
+        "model": os.path.relpath(path, ROOT),
+    }
+    mlp.close()
+    return res
 
 
 if __name__ == "__main__":

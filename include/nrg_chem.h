@@ -131,7 +131,9 @@ typedef struct nrg_chem_opts {
                                   of its pending lanes wait for one (1..100; 0 = default 50)   */
   int32_t waves_per_cu;        /* persistent form: resident waves per CU (0 = occupancy limit) */
   int32_t cell_order;          /* NRG_ORDER_*: order in which the queue hands out cells        */
-  int32_t reserved[4];
+  int32_t use_graph;           /* 1: capture the step's launches in a HIP graph and replay it
+                                  while the arguments stay the same (same results)             */
+  int32_t reserved[3];
 } nrg_chem_opts;
 
 /*
@@ -250,6 +252,36 @@ int nrg_state_equation_dev(nrg_chem_t *h, const nrg_grid *grid,
 int nrg_energy_from_temperature_dev(nrg_chem_t *h, const nrg_grid *grid,
                                     const double *T, const double *const *Y,
                                     const double *const *v, int32_t ndim, double *E_f);
+
+/*
+ * Autotuner of the batched form.  A cell is one lane and a workgroup one
+ * wave (64 cells), so what is tuned is how the queue feeds the waves: launch
+ * form, cell order, the batched-Jacobian threshold, resident waves per CU and
+ * the queue chunk size.  Starting from the handle's current set-up it tries
+ * one change at a time (greedy), each as a full chemistry step on the given
+ * fields timed with HIP events, keeps the fastest and leaves it in the
+ * handle.  Every trial writes the step's outputs (identical for every set-up).
+ * Arguments as nrg_chem_step_dev; `out` receives the trials and the choice.
+ */
+#define NRG_TUNE_MAX_TRIALS 24
+typedef struct nrg_chem_tune_trial {
+  int32_t form, cell_order, jac_batch_pct, waves_per_cu;
+  int64_t chunk_cells;
+  double kernel_ms;
+} nrg_chem_tune_trial;
+
+typedef struct nrg_chem_tuning {
+  nrg_chem_tune_trial best;
+  int32_t n_trials;
+  int32_t reserved;
+  nrg_chem_tune_trial trials[NRG_TUNE_MAX_TRIALS];
+} nrg_chem_tuning;
+
+int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt,
+                          const double *T, const double *p, const double *rho,
+                          const double *const *Y, const int8_t *bc,
+                          double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                          nrg_chem_tuning *out);
 
 /* HIP stream the handle launches on (a hipStream_t), for callers that time or order work. */
 void *nrg_chem_stream(nrg_chem_t *h);

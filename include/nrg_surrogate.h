@@ -14,6 +14,14 @@
  *      of one split step, (dT, dY_1..dY_S)_j = y_j * out_scale_j + out_shift_j;
  *   3. the gate accepts a cell only if
  *        - T, p, dt lie inside the trained domain,
+ *        - cells the network sees as inert (|dT| <= dT_freeze and every
+ *          |dY_s| <= dY_freeze) are accepted with a zero increment, so network
+ *          noise below its resolution never enters the state (a perturbed
+ *          radical pool would make the FP64 integrator resolve a stiff
+ *          relaxation back to the slow manifold in later steps);
+ *        - the network's |dT| is at most dT_max (confidence: fast chemistry,
+ *          where a wrong reaction progress conserves elements and energy and
+ *          so passes the checks below, goes to the FP64 integrator),
  *        - the raw dY conserves every element to elem_tol (relative to the
  *          atoms it moves) + elem_floor [mol/kg],
  *        - after projecting dY onto the element-conserving subspace, no
@@ -57,6 +65,8 @@ typedef struct nrg_surrogate_gate {
   double elem_tol, elem_floor;
   double y_neg_tol;
   double energy_tol;
+  double dT_max;                                     /* K                              */
+  double dT_freeze, dY_freeze;                       /* K, mass fraction (0: off)      */
   int32_t n_elements;                                /* rows of `elements` (<= 4)      */
   int32_t reserved[7];
   double elements[NRG_SUR_MAX_ELEMENTS][32];         /* atoms of element e in species s */
@@ -64,7 +74,8 @@ typedef struct nrg_surrogate_gate {
 
 typedef struct nrg_surrogate_stats {
   int64_t cells_inner, cells_gated, cells_accepted, cells_fallback;
-  int64_t reject_domain, reject_elements, reject_negative, reject_energy;
+  int64_t reject_domain, reject_elements, reject_negative, reject_energy, reject_confidence;
+  int64_t cells_frozen;       /* accepted with a zero increment (inert)                 */
   double infer_ms;            /* featurise + network, HIP events on the handle's stream */
   double gate_ms;
   double fallback_ms;         /* FP64 integration of the rejected cells                */

@@ -78,7 +78,8 @@ class NrgChemOpts(C.Structure):
         ("jac_batch_pct", C.c_int32),
         ("waves_per_cu", C.c_int32),
         ("cell_order", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("use_graph", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -135,7 +136,9 @@ class NrgSurrogateGate(C.Structure):
     _fields_ = [("t_min", C.c_double), ("t_max", C.c_double), ("p_min", C.c_double),
                 ("p_max", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
                 ("elem_tol", C.c_double), ("elem_floor", C.c_double), ("y_neg_tol", C.c_double),
-                ("energy_tol", C.c_double), ("n_elements", C.c_int32), ("reserved", C.c_int32 * 7),
+                ("energy_tol", C.c_double), ("dT_max", C.c_double),
+                ("dT_freeze", C.c_double), ("dY_freeze", C.c_double),
+                ("n_elements", C.c_int32), ("reserved", C.c_int32 * 7),
                 ("elements", (C.c_double * 32) * NRG_SUR_MAX_ELEMENTS)]
 
 
@@ -145,8 +148,25 @@ class NrgSurrogateStats(C.Structure):
                 ("cells_accepted", C.c_int64), ("cells_fallback", C.c_int64),
                 ("reject_domain", C.c_int64), ("reject_elements", C.c_int64),
                 ("reject_negative", C.c_int64), ("reject_energy", C.c_int64),
+                ("reject_confidence", C.c_int64), ("cells_frozen", C.c_int64),
                 ("infer_ms", C.c_double), ("gate_ms", C.c_double), ("fallback_ms", C.c_double),
                 ("fallback", NrgChemStats)]
+
+
+class NrgChemTuneTrial(C.Structure):
+    _fields_ = [("form", C.c_int32), ("cell_order", C.c_int32), ("jac_batch_pct", C.c_int32),
+                ("waves_per_cu", C.c_int32), ("chunk_cells", C.c_int64), ("kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+NRG_TUNE_MAX_TRIALS = 24
+
+
+class NrgChemTuning(C.Structure):
+    _fields_ = [("best", NrgChemTuneTrial), ("n_trials", C.c_int32), ("reserved", C.c_int32),
+                ("trials", NrgChemTuneTrial * NRG_TUNE_MAX_TRIALS)]
 
 
 def default_opts(**kw):
@@ -164,6 +184,7 @@ def default_opts(**kw):
     o.jac_batch_pct = 0
     o.waves_per_cu = 0
     o.cell_order = NRG_ORDER_NATURAL
+    o.use_graph = 0
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -294,6 +315,8 @@ def load_library(path=None):
     lib.nrg_energy_from_temperature_dev.restype = C.c_int
     lib.nrg_chem_stream.argtypes = [vp]
     lib.nrg_chem_stream.restype = vp
+    lib.nrg_chem_autotune_dev.argtypes = step_args[:-2] + [C.POINTER(NrgChemTuning)]
+    lib.nrg_chem_autotune_dev.restype = C.c_int
     # include/nrg_mlp.h
     lib.nrg_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                    C.POINTER(vp)]

@@ -485,6 +485,18 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinArgs a)
   if (threadIdx.x < NBIN && h[threadIdx.x]) atomicAdd(&a.count[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
+// cursor[b] = number of cells in the bins more expensive than b (one wave).
+__global__ void bin_offsets_kernel(BinArgs a)
+{
+  if (threadIdx.x == 0) {
+    unsigned long long pos = 0;
+    for (int k = NBIN - 1; k >= 0; --k) {
+      a.cursor[k] = pos;
+      pos += a.count[k];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) bin_scatter_kernel(BinArgs a)
 {
   __shared__ unsigned int h[NBIN];
@@ -508,7 +520,8 @@ __global__ void __launch_bounds__(256) bin_scatter_kernel(BinArgs a)
 constexpr int NIN = S + 3, NOUT = S + 1;
 
 enum { SUR_FALLBACK = 0, SUR_ACCEPT, SUR_GATED, SUR_R_DOMAIN, SUR_R_ELEM, SUR_R_NEG, SUR_R_ENERGY,
-       SUR_SLOTS = 8 };
+       SUR_R_CONF, SUR_FROZEN, SUR_SLOTS = 9 };
+constexpr int SUR_NREASONS = SUR_R_CONF - SUR_R_DOMAIN + 1;
 
 struct SurArgs {
   GridIdx g;
@@ -521,7 +534,7 @@ struct SurArgs {
   float y_floor;
   double t_min, t_max, p_min, p_max;
   int dt_ok;
-  double elem_tol, elem_floor, y_neg_tol, energy_tol;
+  double elem_tol, elem_floor, y_neg_tol, energy_tol, dT_max, dT_freeze, dY_freeze;
   int ne;
   double E[NRG_SUR_MAX_ELEMENTS][S];
   bf16 *X;                    // ninner x MLP_IN_PAD
@@ -624,12 +637,31 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
       dy[s] = (double)yp[1 + s] * a.out_scale[1 + s] + a.out_shift[1 + s];
       finite = finite && isfinite(dy[s]);
     }
-    if (!finite || !(T >= a.t_min && T <= a.t_max && p >= a.p_min && p <= a.p_max) || !a.dt_ok)
+    bool frozen = false;
+    if (!finite || !(T >= a.t_min && T <= a.t_max && p >= a.p_min && p <= a.p_max) || !a.dt_ok) {
       reason = SUR_R_DOMAIN;
+    } else {
+      double dymax = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) dymax = fmax(dymax, fabs(dy[s]));
+      frozen = fabs(dT) <= a.dT_freeze && dymax <= a.dY_freeze;
+      if (!frozen && !(fabs(dT) <= a.dT_max)) reason = SUR_R_CONF;
+    }
+    if (frozen) {
+      // inert: the FP64 step's outputs for a cell whose concentrations do not change
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        a.Yp[s][off] = 0.0;
+        a.Yp2[s][off] = 0.0;
+      }
+      a.Efp[off] = 0.0;
+      if (a.status) a.status[off] = NRG_CELL_SURROGATE;
+    }
+    if (frozen) reason = -1;
     // element balance of the raw increment: sum_s E_es dY_s / W_s  [mol/kg]
     double res[NRG_SUR_MAX_ELEMENTS] = {0.0, 0.0, 0.0, 0.0};
     double d[S];
-    if (!reason) {
+    if (reason == 0) {
       for (int e = 0; e < a.ne; ++e) {
         double r = 0.0, sc = 0.0;
 #pragma unroll
@@ -641,7 +673,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
         if (fabs(r) > a.elem_tol * sc + a.elem_floor) reason = SUR_R_ELEM;
       }
     }
-    if (!reason) {
+    if (reason == 0) {
       // project onto sum_s E_es dY_s / W_s = 0, weighting the correction by |dY_s|
       double M[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS] = {};
 #pragma unroll
@@ -666,7 +698,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
         if (y[s] + dy[s] < -a.y_neg_tol) reason = SUR_R_NEG;
     }
     double dc[S], efp = 0.0;
-    if (!reason) {
+    if (reason == 0) {
       // chemistry sources of the projected increment, as the FP64 step writes them
 #pragma unroll
       for (int s = 0; s < S; ++s) {
@@ -704,7 +736,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
       const double T1 = calculate_temperature(mc, T, Ef * wmix, X);
       if (!(fabs(dT - (T1 - T)) <= a.energy_tol)) reason = SUR_R_ENERGY;
     }
-    if (!reason) {
+    if (reason == 0) {
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         a.Yp[s][off] = (dc[s] * mc.W[s]) / a.dt;
@@ -714,7 +746,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
       if (a.status) a.status[off] = NRG_CELL_SURROGATE;
     }
   }
-  const bool rej = live && reason != 0;
+  const bool rej = live && reason > 0;
   const unsigned long long mask = __ballot(rej ? 1 : 0);
   if (mask) {
     const int leader = __ffsll((long long)mask) - 1;
@@ -723,16 +755,19 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
     base = __shfl(base, leader, WAVE);
     if (rej) a.list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)q;
   }
-  const unsigned long long n_acc = __popcll(__ballot(live && reason == 0 ? 1 : 0));
+  const unsigned long long n_acc = __popcll(__ballot(live && reason <= 0 ? 1 : 0));
+  const unsigned long long n_frz = __popcll(__ballot(live && reason < 0 ? 1 : 0));
   const unsigned long long n_gat = __popcll(__ballot(gated ? 1 : 0));
-  unsigned long long n_r[4];
+  unsigned long long n_r[SUR_NREASONS];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) n_r[i] = __popcll(__ballot(rej && reason == SUR_R_DOMAIN + i ? 1 : 0));
+  for (int i = 0; i < SUR_NREASONS; ++i)
+    n_r[i] = __popcll(__ballot(rej && reason == SUR_R_DOMAIN + i ? 1 : 0));
   if (lane == 0) {
     if (n_acc) atomicAdd(&a.acc[SUR_ACCEPT], n_acc);
     if (n_gat) atomicAdd(&a.acc[SUR_GATED], n_gat);
+    if (n_frz) atomicAdd(&a.acc[SUR_FROZEN], n_frz);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < SUR_NREASONS; ++i)
       if (n_r[i]) atomicAdd(&a.acc[SUR_R_DOMAIN + i], n_r[i]);
   }
 }
@@ -742,6 +777,14 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 // ===========================================================================
 // Host side
 // ===========================================================================
+// What a captured step graph depends on: its kernel arguments and the queue set-up.
+struct StepKey {
+  StepArgs args;
+  const double *rates;         // buffer the graph's kernels write (reallocated when it grows)
+  int use_hist, form;
+  long long chunk_max;
+};
+
 struct nrg_chem_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -776,6 +819,11 @@ struct nrg_chem_handle {
   long long order_cells = 0;
   bool hist_valid = false;
   GridIdx hist_grid{};
+  // step graph (opts.use_graph)
+  bool use_graph = false;
+  hipGraphExec_t graph_exec = nullptr;
+  StepKey graph_key{};
+  long long graph_captures = 0;
 };
 
 namespace {
@@ -1005,6 +1053,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
   h->k = make_const(o);
   h->form = o.form;
   h->order = o.cell_order;
+  h->use_graph = o.use_graph != 0;
   h->jac_pct = o.jac_batch_pct ? o.jac_batch_pct : 50;
   auto fail = [&](int s) {
     nrg_chem_destroy(h);
@@ -1046,6 +1095,7 @@ void nrg_chem_destroy(nrg_chem_t *h)
   if (h->dev_mech) hipFree(h->dev_mech);
   if (h->dev_acc) hipFree(h->dev_acc);
   if (h->rates) hipFree(h->rates);
+  if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->ev2) hipEventDestroy(h->ev2);
@@ -1065,10 +1115,9 @@ static bool same_grid(const GridIdx &x, const GridIdx &y)
 }
 
 // Build the stiffness-ordered cell list of this call into h->order_list and
-// point a.list / a.hist at it.  One small device->host copy (the bin counts).
-static int build_order(nrg_chem_t *h, StepArgs &a)
+// point a.list / a.hist at it (device work only, so it can be captured).
+static int prepare_order(nrg_chem_t *h, long long n)
 {
-  const long long n = a.g.ninner;
   if (n >= (1ll << 31)) return set_error(NRG_CHEM_E_ARG, "more than 2^31 inner cells");
   if (n > h->order_cells) {
     if (h->order_list) HIP_TRY(hipFree(h->order_list));
@@ -1082,48 +1131,41 @@ static int build_order(nrg_chem_t *h, StepArgs &a)
     h->order_cells = n;
   }
   if (!h->bins) HIP_TRY(hipMalloc(&h->bins, 2 * NBIN * sizeof(unsigned long long)));
+  return NRG_CHEM_OK;
+}
+
+static void enqueue_order(nrg_chem_t *h, StepArgs &a, bool use_hist)
+{
+  const long long n = a.g.ninner;
   BinArgs b{};
   b.g = a.g;
   b.T = a.T;
   b.bc = a.bc;
   b.t_gate = a.k.t_gate;
-  b.hist = (h->hist_valid && same_grid(h->hist_grid, a.g)) ? h->hist : nullptr;
+  b.hist = use_hist ? h->hist : nullptr;
   b.count = h->bins;
   b.cursor = h->bins + NBIN;
   b.list = h->order_list;
   const unsigned nb = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(2 * NBIN), 0, h->stream, h->bins, 2 * NBIN);
   hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, h->stream, b);
-  unsigned long long cnt[NBIN], cur[NBIN];
-  HIP_TRY(hipMemcpyAsync(cnt, h->bins, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  unsigned long long pos = 0;
-  for (int k = NBIN - 1; k >= 0; --k) {
-    cur[k] = pos;
-    pos += cnt[k];
-  }
-  if ((long long)pos != n) return set_error(NRG_CHEM_E_HIP, "cell binning counted %llu of %lld cells", pos, n);
-  HIP_TRY(hipMemcpyAsync(h->bins + NBIN, cur, sizeof(cur), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(bin_offsets_kernel, dim3(1), dim3(64), 0, h->stream, b);
   hipLaunchKernelGGL(bin_scatter_kernel, dim3(nb), dim3(256), 0, h->stream, b);
-  HIP_TRY(hipGetLastError());
-  // the copy from the host array must complete before `cur` goes out of scope
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  a.list = h->order_list;
-  a.hist = h->hist;
-  h->hist_valid = true;
-  h->hist_grid = a.g;
-  return NRG_CHEM_OK;
 }
 
 // Enqueue rates_kernel + chem_bdf_kernel over npos queue positions (all inner
 // cells when a.list is NULL, else the cells a.list names), in chunks of at
 // most chunk_max positions, each with its own queue head.
-static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
+static long long chunk_for(const nrg_chem_t *h, long long npos)
 {
   long long chunk = std::min<long long>(npos, h->chunk_max);
   const long long min_chunk = (npos + MAX_CHUNKS - 1) / MAX_CHUNKS;
-  chunk = std::max<long long>(std::max<long long>(chunk, min_chunk), 1);
-  const long long nchunks = npos > 0 ? (npos + chunk - 1) / chunk : 0;
+  return std::max<long long>(std::max<long long>(chunk, min_chunk), 1);
+}
+
+static int prepare_rates(nrg_chem_t *h, long long npos)
+{
+  const long long chunk = chunk_for(h, npos);
   if (npos > 0 && h->rates_cells < chunk) {
     if (h->rates) HIP_TRY(hipFree(h->rates));
     h->rates = nullptr;
@@ -1131,6 +1173,15 @@ static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
     HIP_TRY(hipMalloc(&h->rates, (size_t)chunk * (2 * R + NM) * sizeof(double)));
     h->rates_cells = chunk;
   }
+  return NRG_CHEM_OK;
+}
+
+static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
+{
+  int st = prepare_rates(h, npos);
+  if (st) return st;
+  const long long chunk = chunk_for(h, npos);
+  const long long nchunks = npos > 0 ? (npos + chunk - 1) / chunk : 0;
   a.rates = h->rates;
   a.rates_out = h->rates;
   for (long long ci = 0; ci < nchunks; ++ci) {
@@ -1180,6 +1231,52 @@ static int integrator_status(const unsigned long long (&acc)[16])
   return NRG_CHEM_OK;
 }
 
+// The device work of one chemistry step: counters reset, optional ordering,
+// rate constants and integrator per chunk.  Launches only.
+static int enqueue_step(nrg_chem_t *h, StepArgs &a, bool use_hist)
+{
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
+  if (a.list) enqueue_order(h, a, use_hist);
+  return enqueue_integrator(h, a, a.g.ninner);
+}
+
+// Replay the captured step when the arguments equal the captured ones;
+// otherwise capture the step into a new graph first.
+static int launch_step_graph(nrg_chem_t *h, StepArgs &a, bool use_hist)
+{
+  StepKey key;
+  std::memset(&key, 0, sizeof(key));
+  std::memcpy(&key.args, &a, sizeof(StepArgs));
+  key.rates = h->rates;
+  key.use_hist = use_hist;
+  key.form = h->form;
+  key.chunk_max = h->chunk_max;
+  if (!h->graph_exec || std::memcmp(&key, &h->graph_key, sizeof(key)) != 0) {
+    if (h->graph_exec) HIP_TRY(hipGraphExecDestroy(h->graph_exec));
+    h->graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    StepArgs c = a;
+    const int st = enqueue_step(h, c, use_hist);
+    const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (st) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    if (e != hipSuccess) return set_error(NRG_CHEM_E_HIP, "graph capture: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&h->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      h->graph_exec = nullptr;
+      return set_error(NRG_CHEM_E_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+    }
+    h->graph_key = key;
+    h->graph_captures++;
+  }
+  HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+  return NRG_CHEM_OK;
+}
+
 static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
                          const double *rho, const double *const *Y, const int8_t *bc,
                          double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
@@ -1213,14 +1310,32 @@ static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const d
   a.acc = h->dev_acc;
   a.jac_pct = h->jac_pct;
   HIP_TRY(hipSetDevice(h->device));
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
-  HIP_TRY(hipGetLastError());
   unsigned long long acc[16] = {};
   float ms = 0.f;
-  if (a.g.ninner > 0) {
+  if (a.g.ninner <= 0) {
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
+    HIP_TRY(hipGetLastError());
+  } else {
+    // buffers first: nothing below allocates, so the sequence can be captured
+    const bool ordered = h->order == NRG_ORDER_STIFFNESS;
+    bool use_hist = false;
+    if (ordered) {
+      if ((st = prepare_order(h, a.g.ninner))) return st;
+      use_hist = h->hist_valid && same_grid(h->hist_grid, a.g);
+      a.list = h->order_list;
+      a.hist = h->hist;
+    }
+    if ((st = prepare_rates(h, a.g.ninner))) return st;
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    if (h->order == NRG_ORDER_STIFFNESS && (st = build_order(h, a))) return st;
-    if ((st = enqueue_integrator(h, a, a.g.ninner))) return st;
+    if (h->use_graph) {
+      if ((st = launch_step_graph(h, a, use_hist))) return st;
+    } else {
+      if ((st = enqueue_step(h, a, use_hist))) return st;
+    }
+    if (ordered) {
+      h->hist_valid = true;
+      h->hist_grid = a.g;
+    }
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipMemcpyAsync(acc, h->dev_acc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1400,6 +1515,100 @@ int nrg_energy_from_temperature_dev(nrg_chem_t *h, const nrg_grid *grid, const d
   return NRG_CHEM_OK;
 }
 
+int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                          const double *p, const double *rho, const double *const *Y,
+                          const int8_t *bc, double *const *Y_prod, double *const *Y_prod2,
+                          double *E_f_prod, nrg_chem_tuning *out)
+{
+  (void)p;
+  if (!h || !out) return set_error(NRG_CHEM_E_ARG, "handle or out is NULL");
+  std::memset(out, 0, sizeof(*out));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, h->device));
+  int occ = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, chem_bdf_kernel<true>, WAVE, 0));
+  occ = std::max(occ, 1);
+  struct Cfg {
+    int form, order, jac, wpc;
+    long long chunk;
+  };
+  auto apply = [&](const Cfg &c) {
+    h->form = c.form;
+    h->order = c.order;
+    h->jac_pct = c.jac;
+    h->persistent_blocks = (c.wpc > 0 ? c.wpc : occ) * prop.multiProcessorCount;
+    h->chunk_max = c.chunk;
+  };
+  auto trial = [&](const Cfg &c, double &ms) -> int {
+    apply(c);
+    nrg_chem_stats s;
+    const int st = step_dev_impl(h, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, nullptr,
+                                 &s, nullptr, nullptr, nullptr);
+    ms = s.kernel_ms;
+    if (st == NRG_CHEM_OK && out->n_trials < NRG_TUNE_MAX_TRIALS) {
+      nrg_chem_tune_trial &t = out->trials[out->n_trials++];
+      t.form = c.form; t.cell_order = c.order; t.jac_batch_pct = c.jac;
+      t.waves_per_cu = c.wpc > 0 ? c.wpc : occ; t.chunk_cells = c.chunk; t.kernel_ms = ms;
+    }
+    return st;
+  };
+  const int wpc0 = h->persistent_blocks / std::max(prop.multiProcessorCount, 1);
+  Cfg best{h->form, h->order, h->jac_pct, wpc0 == occ ? 0 : wpc0, h->chunk_max};
+  double best_ms = 0.0;
+  int st;
+  // the first run sets up the stiffness history that the ordered trials use
+  if ((st = trial(best, best_ms))) return st;
+  if ((st = trial(best, best_ms))) return st;
+  auto consider = [&](Cfg c) -> int {
+    double ms = 0.0;
+    const int s = trial(c, ms);
+    if (s) return s;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+    return NRG_CHEM_OK;
+  };
+  for (int order : {NRG_ORDER_NATURAL, NRG_ORDER_STIFFNESS})
+    if (order != best.order) {
+      Cfg c = best;
+      c.order = order;
+      if ((st = consider(c))) return st;
+    }
+  for (int form : {NRG_FORM_PLAIN, NRG_FORM_PERSISTENT})
+    if (form != best.form) {
+      Cfg c = best;
+      c.form = form;
+      if ((st = consider(c))) return st;
+    }
+  for (int jac : {25, 50, 75, 100})
+    if (jac != best.jac) {
+      Cfg c = best;
+      c.jac = jac;
+      if ((st = consider(c))) return st;
+    }
+  if (best.form == NRG_FORM_PERSISTENT)
+    for (int w : {2 * occ, 4 * occ}) {
+      Cfg c = best;
+      c.wpc = w;
+      if ((st = consider(c))) return st;
+    }
+  for (long long ch : {1ll << 21, 1ll << 22, 1ll << 23, 1ll << 24})
+    if (ch != best.chunk) {
+      Cfg c = best;
+      c.chunk = ch;
+      if ((st = consider(c))) return st;
+    }
+  apply(best);
+  out->best.form = best.form;
+  out->best.cell_order = best.order;
+  out->best.jac_batch_pct = best.jac;
+  out->best.waves_per_cu = best.wpc > 0 ? best.wpc : occ;
+  out->best.chunk_cells = best.chunk;
+  out->best.kernel_ms = best_ms;
+  return NRG_CHEM_OK;
+}
+
 int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surrogate_norm *norm,
                                 const nrg_surrogate_gate *gate, const nrg_grid *grid, double dt,
                                 const double *T, const double *p, const double *rho,
@@ -1453,7 +1662,8 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   sa.p_min = gate->p_min; sa.p_max = gate->p_max;
   sa.dt_ok = dt >= gate->dt_min && dt <= gate->dt_max;
   sa.elem_tol = gate->elem_tol; sa.elem_floor = gate->elem_floor;
-  sa.y_neg_tol = gate->y_neg_tol; sa.energy_tol = gate->energy_tol;
+  sa.y_neg_tol = gate->y_neg_tol; sa.energy_tol = gate->energy_tol; sa.dT_max = gate->dT_max;
+  sa.dT_freeze = gate->dT_freeze; sa.dY_freeze = gate->dY_freeze;
   sa.ne = gate->n_elements;
   for (int e = 0; e < sa.ne; ++e)
     for (int s = 0; s < S; ++s) sa.E[e][s] = gate->elements[e][s];
@@ -1515,6 +1725,8 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     stats->reject_elements = (int64_t)sur[SUR_R_ELEM];
     stats->reject_negative = (int64_t)sur[SUR_R_NEG];
     stats->reject_energy = (int64_t)sur[SUR_R_ENERGY];
+    stats->reject_confidence = (int64_t)sur[SUR_R_CONF];
+    stats->cells_frozen = (int64_t)sur[SUR_FROZEN];
     stats->infer_ms = ms_inf;
     stats->gate_ms = ms_gate;
     stats->fallback_ms = ms_fb;

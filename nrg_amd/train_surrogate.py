@@ -29,6 +29,10 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--calibrate", default=None,
+                    help="only calibrate the gate of this trained file and write it to --out")
+    ap.add_argument("--target-dy", type=float, default=1e-4,
+                    help="max one-step error of accepted major species against FP64")
     a = ap.parse_args(argv)
 
     import torch
@@ -43,6 +47,13 @@ def main(argv=None):
     os.makedirs(a.out, exist_ok=True)
     mech = load_mechanism()
     eng = ChemistryStep(mech)
+    if a.calibrate:
+        net, norm = sg.load(a.calibrate)
+        rep = calibrate(sg, eng, mech, net, norm, a, log)
+        sg.save_with_gate(os.path.join(a.out, "surrogate_keromnes.npz"), net, norm, rep["gate"])
+        with open(os.path.join(a.out, "surrogate_calibration.json"), "w") as fh:
+            json.dump(rep, fh, indent=1)
+        return 0
     t0 = time.time()
     pairs = sg.generate_pairs(eng, mech, a.traj, a.steps, seed=a.seed, groups=a.groups, log=log)
     t_gen = time.time() - t0
@@ -66,13 +77,26 @@ def main(argv=None):
                train_steps=len(tr) // a.batch * a.epochs, final_loss=hist[-1][1],
                samples_per_s=len(tr) * a.epochs / t_train)
     log(json.dumps(rep))
-    sg.save(os.path.join(a.out, "surrogate_keromnes.npz"), net, norm,
-            meta=dict(traj=a.traj, steps=a.steps, epochs=a.epochs, seed=a.seed))
+    cal = calibrate(sg, eng, mech, net, norm, a, log)
+    rep["gate"] = cal["gate"]
+    sg.save_with_gate(os.path.join(a.out, "surrogate_keromnes.npz"), net, norm, cal["gate"],
+                      meta=dict(traj=a.traj, steps=a.steps, epochs=a.epochs, seed=a.seed))
+    with open(os.path.join(a.out, "surrogate_calibration.json"), "w") as fh:
+        json.dump(cal, fh, indent=1)
     with open(os.path.join(a.out, "surrogate_report.json"), "w") as fh:
         json.dump(rep, fh, indent=1)
     net.close()
     eng.close()
     return 0
+
+
+def calibrate(sg, eng, mech, net, norm, a, log):
+    """Gate thresholds for the trained network (see surrogate.calibrate_gate)."""
+    states = sg.calibration_states(eng, mech, 65536, seed=a.seed + 77)
+    best, results = sg.calibrate_gate(eng, mech, net, norm, states, target_dY=a.target_dy, log=log)
+    log("calibrate: chosen %s" % json.dumps(best))
+    return {"gate": {k: best[k] for k in ("dT_max", "energy_tol", "elem_tol", "elem_floor")},
+            "chosen": best, "target_dY": a.target_dy, "candidates": results}
 
 
 if __name__ == "__main__":

@@ -198,3 +198,80 @@ def test_stiffness_order_is_bitwise_identical(mech, engine, form):
     finally:
         eng.close()
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("order", [capi.NRG_ORDER_NATURAL, capi.NRG_ORDER_STIFFNESS])
+def test_graph_replay_is_bitwise_identical(mech, engine, order):
+    """use_graph = 1: the persistent step is captured once per argument set and
+    replayed; every replay (with eager library work and host syncs in between,
+    and an argument change that forces a new capture) equals the eager step."""
+    import torch
+    from nrg_amd.solver import ChemistryStep
+    n = 12000
+    S = mech.species_number
+    st = _to_cuda(inputs.ignition_mix(mech, n, seed=41, frac_window=0.2))
+    eng = ChemistryStep(mech, capi.default_opts(form=capi.NRG_FORM_PERSISTENT, cell_order=order,
+                                                use_graph=1))
+    # fixed buffers, so that equal arguments replay the captured graph
+    Y = st["Y"].contiguous()
+    Yp = torch.zeros((S, n), dtype=torch.float64, device="cuda:0")
+    Yp2 = torch.full((S, n), -7.0, dtype=torch.float64, device="cuda:0")   # as gpu_flat_step
+    Ef = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    status = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    grid = capi.flat_grid(n)
+    try:
+        for rep, dt in enumerate([1e-7, 1e-7, 1e-7, 2e-7, 2e-7, 1e-7]):
+            a = gpu_flat_step(engine, st, dt, counters=False)
+            Yp.fill_(-1.0)
+            Ef.fill_(-1.0)
+            stats = eng.step_dev(grid, dt, st["T"].data_ptr(), st["rho"].data_ptr(),
+                                 [Y[s].data_ptr() for s in range(S)],
+                                 [Yp[s].data_ptr() for s in range(S)],
+                                 [Yp2[s].data_ptr() for s in range(S)], Ef.data_ptr(),
+                                 status=status.data_ptr())
+            torch.cuda.synchronize()
+            _report("graph", order=order, replay=rep, dt=dt, nfe=stats.nfe_sum)
+            assert np.array_equal(a["Y_prod"], Yp.cpu().numpy()), rep
+            assert np.array_equal(a["Y_prod2"], Yp2.cpu().numpy()), rep
+            assert np.array_equal(a["E_f_prod"], Ef.cpu().numpy()), rep
+            assert np.array_equal(a["status"], status.cpu().numpy()), rep
+            assert a["stats"].nfe_sum == stats.nfe_sum
+    finally:
+        eng.close()
+
+
+def test_autotune_picks_fastest_and_keeps_results(mech, engine):
+    """The autotuner's trials all produce the FP64 step's results; the choice
+    is the fastest trial and stays in the handle."""
+    import ctypes as C
+
+    import torch
+    from nrg_amd.solver import ChemistryStep
+    n = 30000
+    S = mech.species_number
+    st = _to_cuda(inputs.ignition_mix(mech, n, seed=43, frac_window=0.1))
+    ref = gpu_flat_step(engine, st, 1e-7, counters=False)
+    eng = ChemistryStep(mech, capi.default_opts(form=capi.NRG_FORM_PERSISTENT))
+    Y = st["Y"].contiguous()
+    Yp = torch.zeros((S, n), dtype=torch.float64, device="cuda:0")
+    Yp2 = torch.zeros((S, n), dtype=torch.float64, device="cuda:0")
+    Ef = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    tun = capi.NrgChemTuning()
+    try:
+        torch.cuda.synchronize()
+        rc = eng.lib.nrg_chem_autotune_dev(
+            eng.h, C.byref(capi.flat_grid(n)), 1e-7, st["T"].data_ptr(), None, st["rho"].data_ptr(),
+            capi.ptr_array([Y[s].data_ptr() for s in range(S)]), None,
+            capi.ptr_array([Yp[s].data_ptr() for s in range(S)]),
+            capi.ptr_array([Yp2[s].data_ptr() for s in range(S)]), Ef.data_ptr(), C.byref(tun))
+        capi.check(eng.lib, rc)
+        trials = [tun.trials[i].as_dict() for i in range(tun.n_trials)]
+        _report("autotune", n_trials=tun.n_trials, best=tun.best.as_dict())
+        assert tun.n_trials >= 8
+        assert tun.best.kernel_ms <= min(t["kernel_ms"] for t in trials[1:]) + 1e-9
+        assert np.array_equal(Yp.cpu().numpy(), ref["Y_prod"])
+        b = gpu_flat_step(eng, st, 1e-7, counters=False)     # runs with the chosen set-up
+        assert np.array_equal(b["Y_prod"], ref["Y_prod"])
+        assert np.array_equal(b["E_f_prod"], ref["E_f_prod"])
+    finally:
+        eng.close()
