@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
     ap.add_argument("--form", choices=["plain", "persistent"], default="persistent",
                     help="integrator launch form (same results)")
+    ap.add_argument("--jacobian", choices=["fd", "analytic"], default="fd",
+                    help="fd: SLATEC finite differences (reference parity); analytic: fast mode")
     ap.add_argument("--order", choices=["natural", "stiffness"], default="natural",
                     help="queue order of the cells (same results)")
     ap.add_argument("--jac-pct", type=int, default=0,
@@ -93,8 +95,9 @@ def main():
         Yf.pr[s].inner(g).copy_(st["Y"][s].view(view))
     form = capi.NRG_FORM_PERSISTENT if args.form == "persistent" else capi.NRG_FORM_PLAIN
     order = capi.NRG_ORDER_STIFFNESS if args.order == "stiffness" else capi.NRG_ORDER_NATURAL
+    jac = capi.NRG_JAC_ANALYTIC if args.jacobian == "analytic" else capi.NRG_JAC_FD_SLATEC
     solver = chemical_kinetics_solver_c(dm, capi.default_opts(form=form, jac_batch_pct=args.jac_pct,
-                                                              cell_order=order))
+                                                              cell_order=order, jacobian=jac))
     E_f = dm.fields["full_energy"]
     solver.engine.energy_from_temperature_dev(g, dm.fields["temperature"].ptr(), Yf.ptrs(),
                                               E_f.ptr())
@@ -160,6 +163,8 @@ def main():
     mean_ms = float(np.mean(kernel_ms))
     mean_flops = float(np.mean(flops))
     achieved_tflops = mean_flops / (mean_ms * 1e-3) / 1e12
+    nchunks = -(-ncells // (1 << 23))
+    tr = profiled_traffic(nchunks)
     roofline = {
         "kernel": "rates_kernel + chem_bdf_kernel (one chemistry step)",
         "bound": "mfma",
@@ -168,7 +173,8 @@ def main():
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tflops / PEAK_FP64_TFLOPS, 6),
-        "traffic": None,
+        "traffic": round(tr["bytes_per_call"]) if tr else None,
+        "traffic_detail": tr,
         "avg_kernel_ms": round(mean_ms, 3),
         "algorithmic_flops_per_launch": mean_flops,
         "hbm_boundary_GBs": round(flop_model.BYTES_BOUNDARY * ncells / (mean_ms * 1e-3) / 1e9, 2),
@@ -229,6 +235,7 @@ def main():
                         "256^3 hot kernels in fresh H2-air, KEROMNES 9sp/24r, dt=1e-7 s",
             "cells": ncells, "dt": args.dt, "mechanism": "KEROMNES",
             "path": "fp64_integrator", "form": args.form, "order": args.order,
+            "jacobian": args.jacobian,
         },
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
@@ -241,6 +248,27 @@ def main():
 
 
 PEAK_BF16_TFLOPS = 2500.0
+HBM_PROFILE = os.path.join(ROOT, "profiles", "chem_step_256cube_hbm_bytes.txt")
+
+
+def profiled_traffic(nchunks, kernels=("chem_bdf_kernel", "rates_kernel")):
+    """HBM bytes of one chemistry call from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (per-dispatch bytes, copy-calibrated) x dispatches per call."""
+    if not os.path.exists(HBM_PROFILE):
+        return None
+    per = {}
+    for line in open(HBM_PROFILE):
+        if line.startswith("TRAFFIC"):
+            _, k, c, b = line.rstrip("\n").split("\t")
+            for want in kernels:
+                if k.startswith(want):
+                    per[(want, c)] = float(b)
+    if not per:
+        return None
+    total = sum(per.values()) * nchunks
+    return {"bytes_per_call": total,
+            "per_dispatch": {"%s %s" % kc: v for kc, v in sorted(per.items())},
+            "dispatches_per_call": nchunks, "source": os.path.relpath(HBM_PROFILE, ROOT)}
 
 
 def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, fp64_value, S):

@@ -1054,4 +1054,64 @@ NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc,
   }
 }
 
+// Net stoichiometric coefficient of species s in reaction r (right - left terms).
+constexpr int nu_of(int r, int s)
+{
+  int n = 0;
+  for (int c = 0; c < 3; ++c) {
+    if (c < topo::nr[r] && topo::right[r][c] == s) ++n;
+    if (c < topo::nl[r] && topo::left[r][c] == s) --n;
+  }
+  return n;
+}
+
+// Fast mode (NRG_JAC_ANALYTIC): the iteration matrix P = I - h*l0*J with the
+// exact J = d(omega)/dC of the law of mass action at the cell's fixed rate
+// constants and [M] (the third-body concentration is frozen per cell, so it
+// has no derivative), as DDPST forms it for a user Jacobian (MITER = 1).  No
+// extra right-hand sides are evaluated; the step sequence therefore differs
+// from the finite-difference (SLATEC MITER = 2) parity mode.
+NRG_DI void jac_analytic(Cell &c, bool mine, const MechCoef &mc, const double *kl,
+                         const double (&Mc)[NM])
+{
+  if (!mine) return;
+  c.nje++;
+  c.ier = false;
+  const double fac = (-el_of(mc, 1, c.nq)) * c.h;
+#pragma unroll
+  for (int q = 0; q < S; ++q)
+#pragma unroll
+    for (int i = 0; i < S; ++i) c.P[q][i] = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const double m = Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int n = side == 0 ? topo::nl[r] : topo::nr[r];
+      const double k = side == 0 ? kl[r * WAVE] : -kl[(R + r) * WAVE];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int sa = side == 0 ? topo::left[r][a] : topo::right[r][a];
+        if (a >= n || sa == S) continue;
+        // d/dC_sa of k * prod_terms: the product of the other terms
+        double d = k;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const int sb = side == 0 ? topo::left[r][b] : topo::right[r][b];
+          if (b < n && b != a) d = d * (sb == S ? m : c.y[sb < S ? sb : 0]);
+        }
+        const double fd = fac * d;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const int nu = nu_of(r, s);
+          if (nu != 0) c.P[sa][s] = c.P[sa][s] + (double)nu * fd;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < S; ++i) c.P[i][i] = c.P[i][i] + 1.0;
+  if (gefa(c.P, c.ipvt) != 0) c.ier = true;
+}
+
 }  // namespace nrg

@@ -300,6 +300,100 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(InferArgs a)
   }
 }
 
+// Inference, hidden width 32*NT with staged weights: 64 rows per wave (two
+// 32-row MFMA tiles share every B fragment), 256 rows per block, and each
+// 16-row k-block of a hidden layer's weights copied once per block into a
+// double-buffered LDS slot that the four waves read.  The per-tile
+// accumulation order equals mlp_infer_kernel's, so the results are identical.
+constexpr int ROWS2 = 64;
+
+template <int NT>
+__global__ void __launch_bounds__(256) mlp_infer2_kernel(InferArgs a)
+{
+  __shared__ bf16 act_all[4][ROWS2][ACT_LD];
+  __shared__ bf16x8 wbuf[2][NT * WAVE];
+  const int w = threadIdx.x / WAVE, l = threadIdx.x & (WAVE - 1);
+  const int r32 = l & 31, hi = l >> 5;
+  bf16 (*act)[ACT_LD] = act_all[w];
+  const long long row0 = (long long)blockIdx.x * (4 * ROWS2) + w * ROWS2;
+  const long long ra = row0 + r32 < a.n ? row0 + r32 : a.n - 1;
+  const long long rb = row0 + 32 + r32 < a.n ? row0 + 32 + r32 : a.n - 1;
+
+  f32x16 acc[2][NT];
+  // layer 0: K = 32 from the packed input; its B fragments (16 KB) from L2
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) acc[0][nt] = acc[1][nt] = f32x16{};
+#pragma unroll
+  for (int kb = 0; kb < MLP_IN_PAD / 16; ++kb) {
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(a.X + ra * MLP_IN_PAD + 16 * kb + 8 * hi);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(a.X + rb * MLP_IN_PAD + 16 * kb + 8 * hi);
+    const bf16x8 *Wb = reinterpret_cast<const bf16x8 *>(a.Wsw[0]) + (kb * NT) * WAVE + l;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const bf16x8 b = Wb[nt * WAVE];
+      acc[0][nt] = mfma(a0, b, acc[0][nt]);
+      acc[1][nt] = mfma(a1, b, acc[1][nt]);
+    }
+  }
+  act_store<NT>(act, acc[0], a.bias[0], l);
+  act_store<NT>(act + 32, acc[1], a.bias[0], l);
+  __syncthreads();
+
+  constexpr int NV = NT * WAVE;            // bf16x8 vectors per k-block
+  static_assert(NV % 256 == 0, "k-block staging assumes whole vectors per thread");
+  for (int layer = 1; layer < a.layers; ++layer) {
+    const bf16x8 *W = reinterpret_cast<const bf16x8 *>(a.Wsw[layer]);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[0][nt] = acc[1][nt] = f32x16{};
+#pragma unroll
+    for (int v = 0; v < NV / 256; ++v) wbuf[0][v * 256 + threadIdx.x] = W[v * 256 + threadIdx.x];
+    __syncthreads();
+    for (int kb = 0; kb < NT * 2; ++kb) {
+      bf16x8 nxt[NV / 256];
+      const bool more = kb + 1 < NT * 2;
+      if (more) {
+#pragma unroll
+        for (int v = 0; v < NV / 256; ++v) nxt[v] = W[(kb + 1) * NV + v * 256 + threadIdx.x];
+      }
+      const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(&act[r32][16 * kb + 8 * hi]);
+      const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(&act[32 + r32][16 * kb + 8 * hi]);
+      const bf16x8 *Wb = wbuf[kb & 1] + l;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const bf16x8 b = Wb[nt * WAVE];
+        acc[0][nt] = mfma(a0, b, acc[0][nt]);
+        acc[1][nt] = mfma(a1, b, acc[1][nt]);
+      }
+      if (more) {
+#pragma unroll
+        for (int v = 0; v < NV / 256; ++v) wbuf[(kb + 1) & 1][v * 256 + threadIdx.x] = nxt[v];
+      }
+      __syncthreads();
+    }
+    act_store<NT>(act, acc[0], a.bias[layer], l);
+    act_store<NT>(act + 32, acc[1], a.bias[layer], l);
+    __syncthreads();
+  }
+
+  // output layer: K = hidden, N = 32 (padded); its fragments (16 KB) from L2
+  f32x16 o0 = {}, o1 = {};
+  const bf16x8 *Wo = reinterpret_cast<const bf16x8 *>(a.Wsw[a.layers]);
+  for (int kb = 0; kb < NT * 2; ++kb) {
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(&act[r32][16 * kb + 8 * hi]);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(&act[32 + r32][16 * kb + 8 * hi]);
+    const bf16x8 b = Wo[kb * WAVE + l];
+    o0 = mfma(a0, b, o0);
+    o1 = mfma(a1, b, o1);
+  }
+  const float *bo = a.bias[a.layers];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const long long row = row0 + 8 * (r / 4) + 4 * hi + (r % 4);
+    if (r32 < a.n_out && row < a.n) a.Y[row * a.n_out + r32] = o0[r] + bo[r32];
+    if (r32 < a.n_out && row + 32 < a.n) a.Y[(row + 32) * a.n_out + r32] = o1[r] + bo[r32];
+  }
+}
+
 unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
@@ -486,7 +580,9 @@ int mlp_infer_bf16(nrg_mlp_t *h, long long n, const bf16 *X, float *Y, hipStream
   switch (h->hidden) {
     case 64: hipLaunchKernelGGL(mlp_infer_kernel<2>, grid, block, 0, stream, a); break;
     case 128: hipLaunchKernelGGL(mlp_infer_kernel<4>, grid, block, 0, stream, a); break;
-    case 256: hipLaunchKernelGGL(mlp_infer_kernel<8>, grid, block, 0, stream, a); break;
+    case 256:
+      hipLaunchKernelGGL(mlp_infer2_kernel<8>, dim3(blocks_for(n, 4 * ROWS2)), block, 0, stream, a);
+      break;
     default: return set_error(NRG_CHEM_E_ARG, "hidden width %d has no inference kernel", h->hidden);
   }
   HIP_TRY(hipGetLastError());

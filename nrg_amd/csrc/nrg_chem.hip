@@ -225,8 +225,12 @@ NRG_DI void finish_cell(const StepArgs &a, const MechCoef &mc, long long q, long
 // Each iteration either evaluates f for the lanes that asked, or -- when at
 // least jac_pct % of the pending lanes wait for a Jacobian, or nothing else is
 // pending -- runs one batched FD Jacobian + LU for the waiting lanes.
-template <bool PERSISTENT>
-__global__ void __launch_bounds__(WAVE) chem_bdf_kernel(StepArgs a)
+#ifndef NRG_BDF_WAVES_PER_EU
+#define NRG_BDF_WAVES_PER_EU 1   // resident waves per SIMD the register budget is sized for
+#endif
+template <bool PERSISTENT, bool ANALYTIC>
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(NRG_BDF_WAVES_PER_EU)))
+chem_bdf_kernel(StepArgs a)
 {
   __shared__ MechCoef mc;
   __shared__ double krate[2 * R * WAVE];
@@ -278,8 +282,12 @@ __global__ void __launch_bounds__(WAVE) chem_bdf_kernel(StepArgs a)
     const int nj = __popcll(__ballot(wantj ? 1 : 0));
     const int nf = __popcll(__ballot(wantf ? 1 : 0));
     const bool run_jac = nj > 0 && (nf == 0 || nj * 100 >= a.jac_pct * (nj + nf));
-    if (run_jac)
-      jac_batch(c, wantj, a.k, mc, kl, Mc);
+    if (run_jac) {
+      if (ANALYTIC)
+        jac_analytic(c, wantj, mc, kl, Mc);
+      else
+        jac_batch(c, wantj, a.k, mc, kl, Mc);
+    }
     else if (wantf)
       rhs(kl, Mc, c.y, c.save2);
     if (have && (c.req == REQ_NONE || (run_jac ? wantj : wantf))) {
@@ -1030,7 +1038,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     nrg_chem_default_opts(&o);
   if (!(o.eps > 0.0) || !(o.ewt > 0.0) || !(o.hmax > 0.0) || o.mxstep <= 0 || o.mxord <= 0)
     return set_error(NRG_CHEM_E_ARG, "invalid integrator options");
-  if (o.jacobian != NRG_JAC_FD_SLATEC)
+  if (o.jacobian != NRG_JAC_FD_SLATEC && o.jacobian != NRG_JAC_ANALYTIC)
     return set_error(NRG_CHEM_E_ARG, "jacobian mode %d not available", o.jacobian);
   if (o.form != NRG_FORM_PLAIN && o.form != NRG_FORM_PERSISTENT)
     return set_error(NRG_CHEM_E_ARG, "unknown form %d", o.form);
@@ -1069,7 +1077,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     hipDeviceProp_t prop;
     int per_cu = 0;
     if (hipGetDeviceProperties(&prop, h->device) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, chem_bdf_kernel<true>, WAVE, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (chem_bdf_kernel<true, false>), WAVE, 0) !=
             hipSuccess)
       return fail(set_error(NRG_CHEM_E_HIP, "occupancy query"));
     if (o.waves_per_cu > 0) per_cu = o.waves_per_cu;
@@ -1191,11 +1199,18 @@ static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
     hipLaunchKernelGGL(rates_kernel, dim3((unsigned)((a.nchunk + 255) / 256)), dim3(256), 0,
                        h->stream, a);
     const long long need = (a.nchunk + WAVE - 1) / WAVE;
+    const bool an = h->opts.jacobian == NRG_JAC_ANALYTIC;
     if (h->form == NRG_FORM_PERSISTENT) {
       const long long nb = std::min<long long>(h->persistent_blocks, need);
-      hipLaunchKernelGGL(chem_bdf_kernel<true>, dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
+      if (an)
+        hipLaunchKernelGGL((chem_bdf_kernel<true, true>), dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
+      else
+        hipLaunchKernelGGL((chem_bdf_kernel<true, false>), dim3((unsigned)nb), dim3(WAVE), 0, h->stream, a);
     } else {
-      hipLaunchKernelGGL(chem_bdf_kernel<false>, dim3((unsigned)need), dim3(WAVE), 0, h->stream, a);
+      if (an)
+        hipLaunchKernelGGL((chem_bdf_kernel<false, true>), dim3((unsigned)need), dim3(WAVE), 0, h->stream, a);
+      else
+        hipLaunchKernelGGL((chem_bdf_kernel<false, false>), dim3((unsigned)need), dim3(WAVE), 0, h->stream, a);
     }
   }
   HIP_TRY(hipGetLastError());
@@ -1526,7 +1541,7 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, h->device));
   int occ = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, chem_bdf_kernel<true>, WAVE, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (chem_bdf_kernel<true, false>), WAVE, 0));
   occ = std::max(occ, 1);
   struct Cfg {
     int form, order, jac, wpc;

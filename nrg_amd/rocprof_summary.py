@@ -8,9 +8,13 @@ prints, per kernel, the average per-dispatch value of each collected counter
 (FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  With ``--calib-mib`` the
 largest device copy kernel (the known-size copy bench.py runs with
 --copy-calibration-mib) is taken as the reference: its expected bytes divided
-by its measured bytes gives the factor applied to the other kernels.
+by its measured bytes gives the factor applied to the other kernels.  Each
+calibrated byte counter also gets a line
+    TRAFFIC <tab> kernel <tab> counter <tab> bytes per dispatch
+which bench.py reads to report the HBM traffic of the chemistry kernels.
 """
 import csv
+import re
 import sys
 from collections import defaultdict
 
@@ -56,11 +60,18 @@ def counters(path, calib_mib=None):
             print("calibration kernel: %s  measured %.1f KiB/dispatch, expected %.1f KiB -> "
                   "factor %.4f" % (_short(best[0]), best[2], expected_kib, expected_kib / best[2]))
     print("%-72s %6s %18s %18s" % ("kernel", "disp", "counter", "avg/dispatch"))
+    traffic = []
     for k in sorted(acc, key=lambda k: -sum(acc[k].values())):
         nd = max(1, len(disp[k]))
         for c, v in sorted(acc[k].items()):
             f = factor.get(c, 1.0)
             print("%-72s %6d %18s %18.1f" % (_short(k), nd, c, v / nd * f))
+            if c in factor:
+                m = re.search(r"([A-Za-z_]\w*_kernel)\b", k)
+                traffic.append("TRAFFIC\t%s\t%s\t%.0f" % (m.group(1) if m else _short(k, 40), c,
+                                                           v / nd * f * 1024.0))
+    for t in traffic:
+        print(t)
 
 
 def main(argv):
