@@ -41,8 +41,11 @@ def parse():
                     help="integrator launch form (same results)")
     ap.add_argument("--jacobian", choices=["fd", "analytic"], default="fd",
                     help="fd: SLATEC finite differences (reference parity); analytic: fast mode")
-    ap.add_argument("--order", choices=["natural", "stiffness"], default="natural",
+    ap.add_argument("--order", choices=["natural", "stiffness"], default="stiffness",
                     help="queue order of the cells (same results)")
+    ap.add_argument("--graph", type=int, default=1, help="capture the step in a HIP graph (same results)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="run the library's autotuner on the initial state before warmup")
     ap.add_argument("--jac-pct", type=int, default=0,
                     help="batched-Jacobian threshold in percent of pending lanes (0: default)")
     ap.add_argument("--surrogate", default="auto",
@@ -97,7 +100,8 @@ def main():
     order = capi.NRG_ORDER_STIFFNESS if args.order == "stiffness" else capi.NRG_ORDER_NATURAL
     jac = capi.NRG_JAC_ANALYTIC if args.jacobian == "analytic" else capi.NRG_JAC_FD_SLATEC
     solver = chemical_kinetics_solver_c(dm, capi.default_opts(form=form, jac_batch_pct=args.jac_pct,
-                                                              cell_order=order, jacobian=jac))
+                                                              cell_order=order, jacobian=jac,
+                                                              use_graph=args.graph))
     E_f = dm.fields["full_energy"]
     solver.engine.energy_from_temperature_dev(g, dm.fields["temperature"].ptr(), Yf.ptrs(),
                                               E_f.ptr())
@@ -117,6 +121,13 @@ def main():
                                                       "full_energy")}
     Y0 = [Yf.pr[s].data.clone() for s in range(S)]
     torch.cuda.synchronize()
+
+    tuning = None
+    if args.autotune:
+        t_tune = time.perf_counter()
+        tun = solver.autotune(args.dt)
+        tuning = {"best": tun.best.as_dict(), "trials": tun.n_trials,
+                  "wall_s": round(time.perf_counter() - t_tune, 2)}
 
     def split_step():
         stats = solver.solve_chemical_kinetics(args.dt)
@@ -235,7 +246,7 @@ def main():
                         "256^3 hot kernels in fresh H2-air, KEROMNES 9sp/24r, dt=1e-7 s",
             "cells": ncells, "dt": args.dt, "mechanism": "KEROMNES",
             "path": "fp64_integrator", "form": args.form, "order": args.order,
-            "jacobian": args.jacobian,
+            "jacobian": args.jacobian, "graph": bool(args.graph), "autotune": tuning,
         },
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,

@@ -1567,6 +1567,14 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
     }
     return st;
   };
+  // trials run eagerly: a capture per trial set-up would add host latency to its time
+  const bool graph = h->use_graph;
+  h->use_graph = false;
+  struct Restore {
+    nrg_chem_t *h;
+    bool g;
+    ~Restore() { h->use_graph = g; }
+  } restore{h, graph};
   const int wpc0 = h->persistent_blocks / std::max(prop.multiProcessorCount, 1);
   Cfg best{h->form, h->order, h->jac_pct, wpc0 == occ ? 0 : wpc0, h->chunk_max};
   double best_ms = 0.0;

@@ -54,7 +54,22 @@ module nrg_chem_binding
     real(c_double) :: kernel_ms
   end type
 
+  ! nrg_chem_autotune_dev: one trial and the table of trials
+  integer(c_int), parameter, public :: NRG_TUNE_MAX_TRIALS = 24
+  type, bind(C), public :: nrg_chem_tune_trial
+    integer(c_int32_t) :: form, cell_order, jac_batch_pct, waves_per_cu
+    integer(c_int64_t) :: chunk_cells
+    real(c_double) :: kernel_ms
+  end type
+
+  type, bind(C), public :: nrg_chem_tuning
+    type(nrg_chem_tune_trial) :: best
+    integer(c_int32_t) :: n_trials, reserved
+    type(nrg_chem_tune_trial) :: trials(NRG_TUNE_MAX_TRIALS)
+  end type
+
   public :: nrg_chem_default_opts, nrg_chem_strerror, nrg_chem_last_error, nrg_chem_abi_version
+  public :: nrg_chem_autotune_dev
   public :: nrg_chem_create, nrg_chem_destroy, nrg_chem_step, nrg_chem_step_dev
   public :: nrg_chem_closure_dev, nrg_state_equation_dev, nrg_energy_from_temperature_dev
   public :: nrg_chem_stream, nrg_chem_message
@@ -149,6 +164,17 @@ module nrg_chem_binding
       type(c_ptr), value :: T, Y, v
       integer(c_int32_t), value :: ndim
       type(c_ptr), value :: E_f
+      integer(c_int) :: status
+    end function
+
+    function nrg_chem_autotune_dev(handle, grid, dt, T, p, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, &
+                                   tuning) bind(C, name="nrg_chem_autotune_dev") result(status)
+      import :: c_int, c_ptr, c_double, nrg_grid, nrg_chem_tuning
+      type(c_ptr), value :: handle
+      type(nrg_grid), intent(in) :: grid
+      real(c_double), value :: dt
+      type(c_ptr), value :: T, p, rho, Y, bc, Y_prod, Y_prod2, E_f_prod
+      type(nrg_chem_tuning), intent(out) :: tuning
       integer(c_int) :: status
     end function
 

@@ -219,6 +219,21 @@ class chemical_kinetics_solver:  # noqa: N801  (reference type name)
                                     bc=self.boundary.ptr(), status=self.cell_status.ptr(),
                                     p=self.p.ptr())
 
+    def autotune(self, time_step):
+        """Run the library's autotuner (nrg_chem_autotune_dev) on the current
+        fields; the chosen queue set-up stays in the handle.  Writes the
+        chemistry outputs (as a step would)."""
+        _order_after_torch()
+        tun = capi.NrgChemTuning()
+        e = self.engine
+        rc = e.lib.nrg_chem_autotune_dev(
+            e.h, C.byref(self.manager.grid), float(time_step), self.T.ptr(), self.p.ptr(),
+            self.rho.ptr(), capi.ptr_array(self.Y.ptrs()), self.boundary.ptr(),
+            capi.ptr_array(self.Y_prod.ptrs()), capi.ptr_array(self.Y_prod2.ptrs()),
+            self.E_f_prod.ptr(), C.byref(tun))
+        check(e.lib, rc)
+        return tun
+
     def apply_split_sources(self, time_step):
         """Legacy CABARET source application + e->T (cabaret_solver.f90:3316-3416 and
         table_approximated_real_gas.f90:410-440), device-resident."""
