@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--order", choices=["natural", "stiffness"], default="stiffness",
                     help="queue order of the cells (same results)")
     ap.add_argument("--graph", type=int, default=1, help="capture the step in a HIP graph (same results)")
+    ap.add_argument("--chunk-log2", type=int, default=0,
+                    help="queue chunk of 2^k cells (0: library default 2^23; the autotuner may change it)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="run the library's autotuner on the initial state before warmup")
     ap.add_argument("--jac-pct", type=int, default=0,
@@ -101,7 +103,8 @@ def main():
     jac = capi.NRG_JAC_ANALYTIC if args.jacobian == "analytic" else capi.NRG_JAC_FD_SLATEC
     solver = chemical_kinetics_solver_c(dm, capi.default_opts(form=form, jac_batch_pct=args.jac_pct,
                                                               cell_order=order, jacobian=jac,
-                                                              use_graph=args.graph))
+                                                              use_graph=args.graph,
+                                                              chunk_log2=args.chunk_log2))
     E_f = dm.fields["full_energy"]
     solver.engine.energy_from_temperature_dev(g, dm.fields["temperature"].ptr(), Yf.ptrs(),
                                               E_f.ptr())
@@ -174,7 +177,9 @@ def main():
     mean_ms = float(np.mean(kernel_ms))
     mean_flops = float(np.mean(flops))
     achieved_tflops = mean_flops / (mean_ms * 1e-3) / 1e12
-    nchunks = -(-ncells // (1 << 23))
+    chunk_cells = (tuning["best"]["chunk_cells"] if tuning else
+                   (1 << (args.chunk_log2 or 23)))
+    nchunks = -(-ncells // chunk_cells)
     tr = profiled_traffic(nchunks)
     roofline = {
         "kernel": "rates_kernel + chem_bdf_kernel (one chemistry step)",
@@ -187,6 +192,7 @@ def main():
         "traffic": round(tr["bytes_per_call"]) if tr else None,
         "traffic_detail": tr,
         "avg_kernel_ms": round(mean_ms, 3),
+        "dispatches_per_call": nchunks,
         "algorithmic_flops_per_launch": mean_flops,
         "hbm_boundary_GBs": round(flop_model.BYTES_BOUNDARY * ncells / (mean_ms * 1e-3) / 1e9, 2),
     }

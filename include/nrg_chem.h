@@ -133,7 +133,9 @@ typedef struct nrg_chem_opts {
   int32_t cell_order;          /* NRG_ORDER_*: order in which the queue hands out cells        */
   int32_t use_graph;           /* 1: capture the step's launches in a HIP graph and replay it
                                   while the arguments stay the same (same results)             */
-  int32_t reserved[3];
+  int32_t chunk_log2;          /* queue chunk = 2^chunk_log2 cells (16..26; 0 = 23): cells per
+                                  rates buffer and per integrator launch                       */
+  int32_t reserved[2];
 } nrg_chem_opts;
 
 /*

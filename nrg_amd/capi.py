@@ -79,7 +79,8 @@ class NrgChemOpts(C.Structure):
         ("waves_per_cu", C.c_int32),
         ("cell_order", C.c_int32),
         ("use_graph", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("chunk_log2", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -185,6 +186,7 @@ def default_opts(**kw):
     o.waves_per_cu = 0
     o.cell_order = NRG_ORDER_NATURAL
     o.use_graph = 0
+    o.chunk_log2 = 0
     for k, v in kw.items():
         setattr(o, k, v)
     return o

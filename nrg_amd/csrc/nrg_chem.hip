@@ -1048,6 +1048,8 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     return set_error(NRG_CHEM_E_ARG, "waves_per_cu must be in 0..32");
   if (o.cell_order != NRG_ORDER_NATURAL && o.cell_order != NRG_ORDER_STIFFNESS)
     return set_error(NRG_CHEM_E_ARG, "unknown cell_order %d", o.cell_order);
+  if (o.chunk_log2 != 0 && (o.chunk_log2 < 16 || o.chunk_log2 > 26))
+    return set_error(NRG_CHEM_E_ARG, "chunk_log2 must be 0 or in 16..26");
   auto *h = new (std::nothrow) nrg_chem_handle();
   if (!h) return set_error(NRG_CHEM_E_ARG, "out of host memory");
   int st = build_mech(mech, h->host_mech);
@@ -1062,6 +1064,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
   h->form = o.form;
   h->order = o.cell_order;
   h->use_graph = o.use_graph != 0;
+  if (o.chunk_log2) h->chunk_max = 1ll << o.chunk_log2;
   h->jac_pct = o.jac_batch_pct ? o.jac_batch_pct : 50;
   auto fail = [&](int s) {
     nrg_chem_destroy(h);

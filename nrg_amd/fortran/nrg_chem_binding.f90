@@ -38,8 +38,8 @@ module nrg_chem_binding
   type, bind(C), public :: nrg_chem_opts
     real(c_double) :: eps, ewt, hmax, t_gate
     integer(c_int32_t) :: mxstep, mxord, jacobian, device, form, jac_batch_pct, waves_per_cu
-    integer(c_int32_t) :: cell_order, use_graph
-    integer(c_int32_t) :: reserved(3)
+    integer(c_int32_t) :: cell_order, use_graph, chunk_log2
+    integer(c_int32_t) :: reserved(2)
   end type
 
   type, bind(C), public :: nrg_grid

@@ -1,9 +1,12 @@
 """Summarise rocprofv3 CSV output for the chemistry kernels.
 
     python -m nrg_amd.rocprof_summary stats   <kernel_stats.csv>
+    python -m nrg_amd.rocprof_summary trace   <kernel_trace.csv> --last N
     python -m nrg_amd.rocprof_summary counter <counter_collection.csv> [--calib-mib M]
 
-``stats`` prints calls / average / total duration per kernel.  ``counter``
+``stats`` prints calls / average / total duration per kernel.  ``trace``
+prints, per kernel, the mean duration of its last N dispatches (the timed
+steps, after warm-up) from the per-dispatch trace.  ``counter``
 prints, per kernel, the average per-dispatch value of each collected counter
 (FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  With ``--calib-mib`` the
 largest device copy kernel (the known-size copy bench.py runs with
@@ -32,6 +35,21 @@ def stats(path):
                                                 float(r["AverageNs"]) / 1e6,
                                                 float(r["TotalDurationNs"]) / 1e6,
                                                 float(r["Percentage"])))
+
+
+def trace(path, last):
+    rows = list(csv.DictReader(open(path)))
+    name_key = "Kernel_Name" if "Kernel_Name" in rows[0] else "Kernel-Name"
+    per = defaultdict(list)
+    for r in rows:
+        b = float(r.get("Start_Timestamp") or r.get("Start-Timestamp"))
+        e = float(r.get("End_Timestamp") or r.get("End-Timestamp"))
+        per[r[name_key]].append((b, e - b))
+    print("%-72s %6s %14s   (mean of the last %d dispatches)" % ("kernel", "disp", "avg_ms", last))
+    for k in sorted(per, key=lambda k: -sum(d for _, d in per[k])):
+        ds = [d for _, d in sorted(per[k])]
+        tail = ds[-last:]
+        print("%-72s %6d %14.3f" % (_short(k), len(ds), sum(tail) / len(tail) / 1e6))
 
 
 def counters(path, calib_mib=None):
@@ -80,6 +98,9 @@ def main(argv):
         return 2
     if argv[0] == "stats":
         stats(argv[1])
+    elif argv[0] == "trace":
+        last = int(argv[argv.index("--last") + 1]) if "--last" in argv else 1
+        trace(argv[1], last)
     elif argv[0] == "counter":
         calib = None
         if "--calib-mib" in argv:
