@@ -66,7 +66,7 @@ typedef struct nrg_surrogate_gate {
   double y_neg_tol;
   double energy_tol;
   double dT_max;                                     /* K                              */
-  double dT_freeze, dY_freeze;                       /* K, mass fraction (0: off)      */
+  double dT_freeze, dY_freeze;                       /* K, mass fraction (dT_freeze 0: off) */
   int32_t n_elements;                                /* rows of `elements` (<= 4)      */
   int32_t reserved[7];
   double elements[NRG_SUR_MAX_ELEMENTS][32];         /* atoms of element e in species s */

@@ -966,9 +966,11 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
 
 // DDPST, MITER = 2 (slatec/ddpst.f90:39-114,142-143) for the lanes with
 // `mine` set: finite-difference Jacobian with adaptive increments FAC,
-// P = I - h*l0*J and its LU.  The column loop is wave-uniform, so the
-// per-column register selections are scalar branches, and one RHS copy
-// serves all nine columns.
+// P = I - h*l0*J and its LU.  The column loop is unrolled: with a static
+// column index the gathers of y_j, YWT_j, FAC_j and the scatter of the column
+// into P are plain register moves instead of select chains over all nine
+// columns (measured: 11-18 % less time per step than one shared loop body,
+// despite nine RHS copies).
 NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc, const double *kl,
                       const double (&Mc)[NM])
 {
@@ -977,7 +979,7 @@ NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc,
     c.ier = false;
   }
   const double l0 = mine ? el_of(mc, 1, c.nq) : 0.0;
-#pragma nounroll
+#pragma unroll
   for (int j = 0; j < S; ++j) {
     double ywj = 0.0, yj = 0.0, fj = 0.0, s2j = 0.0, yh2j = 0.0;
 #pragma unroll

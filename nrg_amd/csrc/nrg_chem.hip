@@ -652,7 +652,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
       double dymax = 0.0;
 #pragma unroll
       for (int s = 0; s < S; ++s) dymax = fmax(dymax, fabs(dy[s]));
-      frozen = fabs(dT) <= a.dT_freeze && dymax <= a.dY_freeze;
+      frozen = a.dT_freeze > 0.0 && fabs(dT) <= a.dT_freeze && dymax <= a.dY_freeze;
       if (!frozen && !(fabs(dT) <= a.dT_max)) reason = SUR_R_CONF;
     }
     if (frozen) {
