@@ -50,6 +50,8 @@ def parse():
                     help="run the library's autotuner on the initial state before warmup")
     ap.add_argument("--jac-pct", type=int, default=0,
                     help="batched-Jacobian threshold in percent of pending lanes (0: default)")
+    ap.add_argument("--gather-pct", type=int, default=0,
+                    help="error-test gathering threshold in percent of stepping lanes (same results)")
     ap.add_argument("--surrogate", default="auto",
                     help="trained surrogate (.npz) timed beside the FP64 integrator from the same "
                          "initial state; 'auto': nrg_amd/data/surrogate_keromnes.npz if present; "
@@ -64,6 +66,19 @@ def parse():
                     help="run one device copy of this size before the timed region, as the "
                          "known-size reference for HBM counters (FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def effective_setup(args, tuning):
+    """Queue set-up the timed steps ran with: the autotuner's choice when it ran."""
+    from nrg_amd import capi
+    if tuning is None:
+        return {"form": args.form, "order": args.order, "jac_batch_pct": args.jac_pct or 50,
+                "errtest_gather_pct": args.gather_pct, "chunk_log2": args.chunk_log2 or 23}
+    b = tuning["best"]
+    return {"form": "persistent" if b["form"] == capi.NRG_FORM_PERSISTENT else "plain",
+            "order": "stiffness" if b["cell_order"] == capi.NRG_ORDER_STIFFNESS else "natural",
+            "jac_batch_pct": b["jac_batch_pct"], "errtest_gather_pct": b["errtest_gather_pct"],
+            "chunk_log2": int(b["chunk_cells"]).bit_length() - 1}
 
 
 def main():
@@ -104,7 +119,8 @@ def main():
     solver = chemical_kinetics_solver_c(dm, capi.default_opts(form=form, jac_batch_pct=args.jac_pct,
                                                               cell_order=order, jacobian=jac,
                                                               use_graph=args.graph,
-                                                              chunk_log2=args.chunk_log2))
+                                                              chunk_log2=args.chunk_log2,
+                                                              errtest_gather_pct=args.gather_pct))
     E_f = dm.fields["full_energy"]
     solver.engine.energy_from_temperature_dev(g, dm.fields["temperature"].ptr(), Yf.ptrs(),
                                               E_f.ptr())
@@ -251,8 +267,8 @@ def main():
             "workload": "FP64 Gear-BDF split step (SLATEC DDRIV3 semantics, FD Jacobian), "
                         "256^3 hot kernels in fresh H2-air, KEROMNES 9sp/24r, dt=1e-7 s",
             "cells": ncells, "dt": args.dt, "mechanism": "KEROMNES",
-            "path": "fp64_integrator", "form": args.form, "order": args.order,
-            "jacobian": args.jacobian, "graph": bool(args.graph), "autotune": tuning,
+            "path": "fp64_integrator", "jacobian": args.jacobian, "graph": bool(args.graph),
+            **effective_setup(args, tuning), "autotune": tuning,
         },
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,

@@ -135,7 +135,11 @@ typedef struct nrg_chem_opts {
                                   while the arguments stay the same (same results)             */
   int32_t chunk_log2;          /* queue chunk = 2^chunk_log2 cells (16..26; 0 = 23): cells per
                                   rates buffer and per integrator launch                       */
-  int32_t reserved[2];
+  int32_t errtest_gather_pct;  /* 0..100: a lane whose corrector has converged waits at the
+                                  error test until this percentage of the wave's lanes that
+                                  are stepping is there (100: the wave's cells take their
+                                  steps' error tests together; 0: no waiting).  Same results. */
+  int32_t reserved;
 } nrg_chem_opts;
 
 /*
@@ -270,6 +274,7 @@ typedef struct nrg_chem_tune_trial {
   int32_t form, cell_order, jac_batch_pct, waves_per_cu;
   int64_t chunk_cells;
   double kernel_ms;
+  int32_t errtest_gather_pct, reserved;
 } nrg_chem_tune_trial;
 
 typedef struct nrg_chem_tuning {

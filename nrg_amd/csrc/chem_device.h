@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "topology_keromnes.h"
 
 #define NRG_DI __device__ __forceinline__
@@ -64,6 +66,7 @@ struct BdfConst {
   double br, bl, bp, facmin;     // u**.875, u**.75, u**-.15, u**.78 (slatec/ddpst.f90:54-57)
   double t_gate;
   int mxstep, maxord;
+  int gather_pct;
 };
 
 NRG_DI double dmax(double a, double b) { return (a >= b) ? a : b; }
@@ -73,6 +76,16 @@ NRG_DI double dmin(double a, double b) { return (a <= b) ? a : b; }
 // which calls pow a few times per step from several sites; inlined, every
 // call site would carry its own 100+-instruction expansion.
 __device__ __noinline__ double nl_pow(double x, double y) { return pow(x, y); }
+
+// Three independent powers in one call: the order-selection candidates of a
+// step, whose dependent chains interleave here instead of running back to back.
+struct Pow3 {
+  double a, b, c;
+};
+__device__ __noinline__ Pow3 nl_pow3(double xa, double ya, double xb, double yb, double xc, double yc)
+{
+  return Pow3{pow(xa, ya), pow(xb, yb), pow(xc, yc)};
+}
 
 // Math for the rate-constant kernel, where inlining is the better trade.
 struct InlineMath {
@@ -132,6 +145,56 @@ constexpr bool uses_m(int r)
   return topo::rtype[r] == 1 || topo::rtype[r] == 2 || topo::rtype[r] == 3 ||
          topo::rtype[r] == 6 || topo::rtype[r] == 7;
 }
+
+// Species that no reaction consumes or produces (N2 in H2-air) enter only
+// through the third-body concentrations, which are fixed per cell.  For them
+// f_s = 0 and no f_i depends on y_s, so under DDRIV3 every correction, every
+// YH column beyond the first and every norm term of theirs is an exact zero
+// (Lawson's norm passes zeros through unchanged) and their row and column of
+// P are those of the identity (the LU and the solves of the other species do
+// not see them).  The integrator state therefore holds the SA active species;
+// an inert species keeps its initial concentration.
+constexpr bool reacts(int s)
+{
+  for (int r = 0; r < R; ++r)
+    for (int c = 0; c < 3; ++c)
+      if ((c < topo::nl[r] && topo::left[r][c] == s) || (c < topo::nr[r] && topo::right[r][c] == s))
+        return true;
+  return false;
+}
+constexpr int count_active()
+{
+  int n = 0;
+  for (int s = 0; s < S; ++s) n += reacts(s) ? 1 : 0;
+  return n;
+}
+constexpr int SA = count_active();
+constexpr int NI = S - SA;
+struct SpeciesMap {
+  int act[S];                    // active slot of species s, or -1
+  int full[SA];                  // species of active slot a
+  int islot[S];                  // inert slot of species s, or -1
+  int inert[NI > 0 ? NI : 1];    // species of inert slot i
+};
+constexpr SpeciesMap make_species_map()
+{
+  SpeciesMap m{};
+  int a = 0, i = 0;
+  for (int s = 0; s < S; ++s) {
+    m.act[s] = -1;
+    m.islot[s] = -1;
+    if (reacts(s)) {
+      m.act[s] = a;
+      m.full[a++] = s;
+    } else {
+      m.islot[s] = i;
+      m.inert[i++] = s;
+    }
+  }
+  return m;
+}
+constexpr SpeciesMap kSp = make_species_map();
+constexpr int act_of(int s) { return (s >= 0 && s < S && kSp.act[s] >= 0) ? kSp.act[s] : 0; }
 
 // ---------------------------------------------------------------------------
 // NASA-7 thermochemistry (thermophysical_properties_class.f90:225-300)
@@ -269,12 +332,13 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
 }
 
 // ---------------------------------------------------------------------------
-// Mass-action right-hand side, the accumulation order of fun (:507-536).
+// Mass-action right-hand side, the accumulation order of fun (:507-536), on
+// the active species (an inert species' f is zero).
 // ---------------------------------------------------------------------------
-NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[S], double (&Zd)[S])
+NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[SA], double (&Zd)[SA])
 {
 #pragma unroll
-  for (int s = 0; s < S; ++s) Zd[s] = 0.0;
+  for (int s = 0; s < SA; ++s) Zd[s] = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     double fwd = kl[r * WAVE];
@@ -282,23 +346,24 @@ NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[S], 
     for (int c = 0; c < 3; ++c) {
       const int sp = topo::left[r][c];
       if (c < topo::nl[r])
-        fwd = fwd * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[sp < S ? sp : 0]);
+        fwd = fwd * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[act_of(sp)]);
     }
     double rev = kl[(R + r) * WAVE];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int sp = topo::right[r][c];
       if (c < topo::nr[r])
-        rev = rev * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[sp < S ? sp : 0]);
+        rev = rev * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[act_of(sp)]);
     }
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
+    for (int a = 0; a < SA; ++a) {
+      const int s = kSp.full[a];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        if (c < topo::nl[r] && topo::left[r][c] == s) Zd[s] = (Zd[s] - fwd) + rev;
+        if (c < topo::nl[r] && topo::left[r][c] == s) Zd[a] = (Zd[a] - fwd) + rev;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        if (c < topo::nr[r] && topo::right[r][c] == s) Zd[s] = (Zd[s] - rev) + fwd;
+        if (c < topo::nr[r] && topo::right[r][c] == s) Zd[a] = (Zd[a] - rev) + fwd;
     }
   }
 }
@@ -307,20 +372,20 @@ NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[S], 
 // DNRM2 (slatec/dnrm2.f90:84-160): Lawson's phases as a per-element state machine.
 // st: 0 = start (label 30), 1 = phase 1 (50), 2 = phase 2 (70), 3 = phase 4 (110),
 //     4 = phase 3 (the DO 95 loop, unscaled sum).
-// One out-of-line copy serves the seven call sites; the vector arrives in
-// argument VGPRs.
+// One out-of-line copy serves the call sites; the vector arrives in argument
+// VGPRs.  The inert species' terms, exact zeros, are left out: in every phase
+// a zero leaves SUM, XMAX and the phase as they were.  HITEST keeps DDRIV3's
+// N (all species).
 // ---------------------------------------------------------------------------
-static_assert(S == 9, "dnrm2_vals takes the 9 species of the compiled mechanism");
-__device__ __noinline__ double dnrm2_vals(double x0, double x1, double x2, double x3, double x4,
-                                          double x5, double x6, double x7, double x8)
+template <int n>
+NRG_DI double dnrm2_core(const double *x)
 {
-  const double x[S] = {x0, x1, x2, x3, x4, x5, x6, x7, x8};
   const double cutlo = 8.232e-11;
   const double hitest = 1.304e19 / (double)S;
   double sum = 0.0, xmax = 0.0;
   int st = 0;
 #pragma unroll
-  for (int i = 0; i < S; ++i) {
+  for (int i = 0; i < n; ++i) {
     const double v = x[i], av = fabs(v);
     bool done = false;
     if (st == 0) {
@@ -386,9 +451,53 @@ __device__ __noinline__ double dnrm2_vals(double x0, double x1, double x2, doubl
   return (st == 4) ? sqrt(sum) : xmax * sqrt(sum);
 }
 
-NRG_DI double dnrm2(const double (&x)[S])
+template <class... D>
+__device__ __noinline__ double dnrm2_vals(D... xs)
 {
-  return dnrm2_vals(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8]);
+  const double x[sizeof...(D)] = {xs...};
+  return dnrm2_core<(int)sizeof...(D)>(x);
+}
+
+// Two norms in one call (independent state machines, interleaved).
+struct Norm2 {
+  double a, b;
+};
+template <class... D>
+__device__ __noinline__ Norm2 dnrm2_pair_vals(D... xs)
+{
+  constexpr int n = (int)sizeof...(D) / 2;
+  const double v[2 * n] = {xs...};
+  return Norm2{dnrm2_core<n>(v), dnrm2_core<n>(v + n)};
+}
+
+template <int N, int... I>
+NRG_DI double dnrm2_seq(const double (&x)[N], std::integer_sequence<int, I...>)
+{
+  return dnrm2_vals(x[I]...);
+}
+
+// Norm of a vector over the active species.
+NRG_DI double dnrm2(const double (&x)[SA])
+{
+  return dnrm2_seq<SA>(x, std::make_integer_sequence<int, SA>{});
+}
+
+template <int... I>
+NRG_DI Norm2 dnrm2_pair_seq(const double (&x)[SA], const double (&y)[SA], std::integer_sequence<int, I...>)
+{
+  return dnrm2_pair_vals(x[I]..., y[I]...);
+}
+
+// Norms of two vectors over the active species.
+NRG_DI Norm2 dnrm2_pair(const double (&x)[SA], const double (&y)[SA])
+{
+  return dnrm2_pair_seq(x, y, std::make_integer_sequence<int, SA>{});
+}
+
+// Norm of a vector over all species.
+NRG_DI double dnrm2_all(const double (&x)[S])
+{
+  return dnrm2_seq<S>(x, std::make_integer_sequence<int, S>{});
 }
 
 // ---------------------------------------------------------------------------
@@ -397,79 +506,135 @@ NRG_DI double dnrm2(const double (&x)[S])
 // ---------------------------------------------------------------------------
 NRG_DI bool wave_any(bool p) { return __any(p ? 1 : 0) != 0; }
 
-NRG_DI int gefa(double (&a)[S][S], int (&ipvt)[S])
+// y[i] += t * x[i] for i in [LO, SA), left alone on the lanes with `skip` set
+// (DAXPY returns at once for DA == 0, slatec/daxpy.f90:68; adding 0*x is not
+// the same when x is not finite or y is -0).  The wave takes the select-free
+// form when no lane skips and drops the loop when every lane does; both
+// happen for whole columns (a species that no rate depends on gives exact
+// zeros in P and in the corrector's right-hand side).
+template <int LO>
+NRG_DI void daxpy_lanes(bool skip, double t, const double (&x)[SA], double (&y)[SA])
 {
-  int info = 0;
+  if (!wave_any(!skip)) return;
+  if (wave_any(skip)) {
 #pragma unroll
-  for (int k = 0; k < S - 1; ++k) {
-    int l = k;
-    double dmx = fabs(a[k][k]), piv = a[k][k];
+    for (int i = LO; i < SA; ++i) y[i] = skip ? y[i] : y[i] + t * x[i];
+  } else {
 #pragma unroll
-    for (int i = k + 1; i < S; ++i) {
-      const double xm = fabs(a[k][i]);
-      if (xm > dmx) {
-        l = i;
-        dmx = xm;
-        piv = a[k][i];
-      }
-    }
-    ipvt[k] = l;
-    const bool zero = (piv == 0.0);
-    if (zero) info = k + 1;
-    const bool anyswap = wave_any(l != k);
-    if (anyswap) {
+    for (int i = LO; i < SA; ++i) y[i] = y[i] + t * x[i];
+  }
+}
+
+// As daxpy_lanes over [0, HI).
+template <int HI>
+NRG_DI void daxpy_head(bool skip, double t, const double (&x)[SA], double (&y)[SA])
+{
+  if (!wave_any(!skip)) return;
+  if (wave_any(skip)) {
 #pragma unroll
-      for (int i = k + 1; i < S; ++i) a[k][i] = (l == i) ? a[k][k] : a[k][i];
-      a[k][k] = piv;
-    }
-    const double t = -1.0 / a[k][k];
+    for (int i = 0; i < HI; ++i) y[i] = skip ? y[i] : y[i] + t * x[i];
+  } else {
 #pragma unroll
-    for (int i = k + 1; i < S; ++i) a[k][i] = zero ? a[k][i] : t * a[k][i];
+    for (int i = 0; i < HI; ++i) y[i] = y[i] + t * x[i];
+  }
+}
+
+template <int K>
+NRG_DI void gefa_col(double (&a)[SA][SA], int (&ipvt)[SA], int &info)
+{
+  int l = K;
+  double dmx = fabs(a[K][K]), piv = a[K][K];
 #pragma unroll
-    for (int j = k + 1; j < S; ++j) {
-      double tj = a[j][k];
-      if (anyswap) {
-#pragma unroll
-        for (int i = k + 1; i < S; ++i) tj = (l == i) ? a[j][i] : tj;
-#pragma unroll
-        for (int i = k + 1; i < S; ++i) a[j][i] = (l == i) ? a[j][k] : a[j][i];
-        a[j][k] = tj;
-      }
-      const bool skip = zero || (tj == 0.0);
-#pragma unroll
-      for (int i = k + 1; i < S; ++i) a[j][i] = skip ? a[j][i] : a[j][i] + tj * a[k][i];
+  for (int i = K + 1; i < SA; ++i) {
+    const double xm = fabs(a[K][i]);
+    if (xm > dmx) {
+      l = i;
+      dmx = xm;
+      piv = a[K][i];
     }
   }
-  ipvt[S - 1] = S - 1;
-  if (a[S - 1][S - 1] == 0.0) info = S;
+  ipvt[K] = l;
+  const bool zero = (piv == 0.0);
+  if (zero) info = K + 1;
+  const bool anyswap = wave_any(l != K);
+  if (anyswap) {
+#pragma unroll
+    for (int i = K + 1; i < SA; ++i) a[K][i] = (l == i) ? a[K][K] : a[K][i];
+    a[K][K] = piv;
+  }
+  // DSCAL of the column below the pivot (skipped for a zero pivot)
+  const double t = -1.0 / a[K][K];
+  if (wave_any(zero)) {
+#pragma unroll
+    for (int i = K + 1; i < SA; ++i) a[K][i] = zero ? a[K][i] : t * a[K][i];
+  } else {
+#pragma unroll
+    for (int i = K + 1; i < SA; ++i) a[K][i] = t * a[K][i];
+  }
+#pragma unroll
+  for (int j = K + 1; j < SA; ++j) {
+    double tj = a[j][K];
+    if (anyswap) {
+#pragma unroll
+      for (int i = K + 1; i < SA; ++i) tj = (l == i) ? a[j][i] : tj;
+#pragma unroll
+      for (int i = K + 1; i < SA; ++i) a[j][i] = (l == i) ? a[j][K] : a[j][i];
+      a[j][K] = tj;
+    }
+    daxpy_lanes<K + 1>(zero || (tj == 0.0), tj, a[K], a[j]);
+  }
+}
+
+template <int... K>
+NRG_DI void gefa_cols(double (&a)[SA][SA], int (&ipvt)[SA], int &info, std::integer_sequence<int, K...>)
+{
+  (gefa_col<K>(a, ipvt, info), ...);
+}
+
+NRG_DI int gefa(double (&a)[SA][SA], int (&ipvt)[SA])
+{
+  int info = 0;
+  gefa_cols(a, ipvt, info, std::make_integer_sequence<int, SA - 1>{});
+  ipvt[SA - 1] = SA - 1;
+  if (a[SA - 1][SA - 1] == 0.0) info = SA;
   return info;
 }
 
-NRG_DI void gesl(const double (&a)[S][S], const int (&ipvt)[S], double (&b)[S])
+template <int K>
+NRG_DI void gesl_fwd(const double (&a)[SA][SA], const int (&ipvt)[SA], double (&b)[SA])
 {
+  const int l = ipvt[K];
+  double t = b[K];
+  if (wave_any(l != K)) {
 #pragma unroll
-  for (int k = 0; k < S - 1; ++k) {
-    const int l = ipvt[k];
-    double t = b[k];
-    if (wave_any(l != k)) {
+    for (int i = K + 1; i < SA; ++i) t = (l == i) ? b[i] : t;
 #pragma unroll
-      for (int i = k + 1; i < S; ++i) t = (l == i) ? b[i] : t;
-#pragma unroll
-      for (int i = k + 1; i < S; ++i) b[i] = (l == i) ? b[k] : b[i];
-      b[k] = t;
-    }
-    const bool skip = (t == 0.0);
-#pragma unroll
-    for (int i = k + 1; i < S; ++i) b[i] = skip ? b[i] : b[i] + t * a[k][i];
+    for (int i = K + 1; i < SA; ++i) b[i] = (l == i) ? b[K] : b[i];
+    b[K] = t;
   }
-#pragma unroll
-  for (int k = S - 1; k >= 0; --k) {
-    b[k] = b[k] / a[k][k];
-    const double t = -b[k];
-    const bool skip = (t == 0.0);
-#pragma unroll
-    for (int i = 0; i < k; ++i) b[i] = skip ? b[i] : b[i] + t * a[k][i];
-  }
+  daxpy_lanes<K + 1>(t == 0.0, t, a[K], b);
+}
+
+template <int K>
+NRG_DI void gesl_back(const double (&a)[SA][SA], double (&b)[SA])
+{
+  b[K] = b[K] / a[K][K];
+  const double t = -b[K];
+  daxpy_head<K>(t == 0.0, t, a[K], b);
+}
+
+template <int... K>
+NRG_DI void gesl_steps(const double (&a)[SA][SA], const int (&ipvt)[SA], double (&b)[SA],
+                       std::integer_sequence<int, K...>)
+{
+  (gesl_fwd<K>(a, ipvt, b), ...);
+  (gesl_back<SA - 1 - K>(a, b), ...);
+  gesl_back<SA - 1 - (int)sizeof...(K)>(a, b);
+}
+
+NRG_DI void gesl(const double (&a)[SA][SA], const int (&ipvt)[SA], double (&b)[SA])
+{
+  gesl_steps(a, ipvt, b, std::make_integer_sequence<int, SA - 1>{});
 }
 
 // ---------------------------------------------------------------------------
@@ -495,10 +660,11 @@ enum CellOutcome { OUT_OK = 0, OUT_MXSTEP = 2, OUT_STEP_ZERO = 3, OUT_REPEATED_F
                    OUT_SINGULAR = 5, OUT_EPS = 6 };
 
 struct Cell {
-  double yh[MAXORD + 1][S];   // yh[j][i] = YH(i, j+1)
-  double y[S], ywt[S], save1[S], save2[S], fac[S];
-  double P[S][S];             // iteration matrix / LU factors, column-major
-  int ipvt[S];
+  double yh[MAXORD + 1][SA];   // yh[j][i] = YH(i, j+1)
+  double y[SA], ywt[SA], save1[SA], save2[SA], fac[SA];
+  double P[SA][SA];             // iteration matrix / LU factors, column-major
+  int ipvt[SA];
+  double yi[NI > 0 ? NI : 1];   // concentrations of the inert species (constant)
   double t, h, rc, rmax, trend, told, d1, ctest;
   int nq, nwait, jtask, jstepl, nstep, nfe, nje, ncorr, natt;
   int ntry, nfail, iter, pc, req, outcome;
@@ -513,7 +679,7 @@ NRG_DI double tq_of(const MechCoef &mc, int k, int nq) { return mc.tq[nq - 1][k 
 
 // DDPSC (slatec/ddpsc.f90:26-38) for a compile-time order.
 template <int NQ, bool ADD>
-NRG_DI void psc_n(double (&yh)[MAXORD + 1][S])
+NRG_DI void psc_n(double (&yh)[MAXORD + 1][SA])
 {
 #pragma unroll
   for (int j1 = 1; j1 <= NQ; ++j1)
@@ -521,12 +687,12 @@ NRG_DI void psc_n(double (&yh)[MAXORD + 1][S])
     for (int j2 = j1; j2 <= NQ; ++j2) {
       const int j = NQ - j2 + j1;
 #pragma unroll
-      for (int i = 0; i < S; ++i) yh[j - 1][i] = ADD ? yh[j - 1][i] + yh[j][i] : yh[j - 1][i] - yh[j][i];
+      for (int i = 0; i < SA; ++i) yh[j - 1][i] = ADD ? yh[j - 1][i] + yh[j][i] : yh[j - 1][i] - yh[j][i];
     }
 }
 
 template <bool ADD>
-NRG_DI void psc(int nq, double (&yh)[MAXORD + 1][S])
+NRG_DI void psc(int nq, double (&yh)[MAXORD + 1][SA])
 {
   switch (nq) {
     case 1: psc_n<1, ADD>(yh); break;
@@ -538,10 +704,10 @@ NRG_DI void psc(int nq, double (&yh)[MAXORD + 1][S])
 }
 
 // YH(:, col+1) read through a select chain (col in 1..MAXORD).
-NRG_DI void yh_col(const Cell &c, int col, double (&out)[S])
+NRG_DI void yh_col(const Cell &c, int col, double (&out)[SA])
 {
 #pragma unroll
-  for (int i = 0; i < S; ++i) {
+  for (int i = 0; i < SA; ++i) {
     double v = opaque(c.yh[1][i]);
 #pragma unroll
     for (int j = 2; j <= MAXORD; ++j) v = (col == j) ? opaque(c.yh[j][i]) : v;
@@ -563,7 +729,7 @@ NRG_DI void scl(Cell &c, double rh, const BdfConst &k)
     r1 = r1 * rh;
     const bool on = (j <= c.nq);
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.yh[j][i] = on ? c.yh[j][i] * r1 : c.yh[j][i];
+    for (int i = 0; i < SA; ++i) c.yh[j][i] = on ? c.yh[j][i] * r1 : c.yh[j][i];
   }
   c.h = c.h * rh;
   c.rc = c.rc * rh;
@@ -573,10 +739,10 @@ NRG_DI void scl(Cell &c, double rh, const BdfConst &k)
 NRG_DI double cor(Cell &c, const BdfConst &k)
 {
 #pragma unroll
-  for (int i = 0; i < S; ++i) c.save2[i] = ((c.h * c.save2[i]) - c.yh[1][i]) - c.save1[i];
+  for (int i = 0; i < SA; ++i) c.save2[i] = ((c.h * c.save2[i]) - c.yh[1][i]) - c.save1[i];
   gesl(c.P, c.ipvt, c.save2);
 #pragma unroll
-  for (int i = 0; i < S; ++i) {
+  for (int i = 0; i < SA; ++i) {
     c.save1[i] = c.save1[i] + c.save2[i];
     c.save2[i] = c.save2[i] / dmax(fabs(c.y[i]), c.ywt[i]);
   }
@@ -587,23 +753,23 @@ NRG_DI double cor(Cell &c, const BdfConst &k)
 // DDNTP, K = 0 (slatec/ddntp.f90:25-32).
 NRG_DI void ntp(Cell &c, double tout)
 {
-  double col[S];
+  double col[SA];
   yh_col(c, c.nq, col);
 #pragma unroll
-  for (int i = 0; i < S; ++i) c.y[i] = col[i];
+  for (int i = 0; i < SA; ++i) c.y[i] = col[i];
   const double r = (tout - c.t) / c.h;
 #pragma unroll
   for (int j = MAXORD; j >= 1; --j) {
     const bool on = (j <= c.nq);
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.y[i] = on ? c.yh[j - 1][i] + r * c.y[i] : c.y[i];
+    for (int i = 0; i < SA; ++i) c.y[i] = on ? c.yh[j - 1][i] + r * c.y[i] : c.y[i];
   }
 }
 
 NRG_DI void copy_yh0_to_y(Cell &c)
 {
 #pragma unroll
-  for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i];
+  for (int i = 0; i < SA; ++i) c.y[i] = c.yh[0][i];
 }
 
 NRG_DI void finish(Cell &c, int outcome)
@@ -652,7 +818,7 @@ NRG_DI void bdf_init(Cell &c, double tout, const BdfConst &k)
   c.ntry = 0;
   c.nfail = 0;
 #pragma unroll
-  for (int i = 0; i < S; ++i) c.yh[0][i] = c.y[i];
+  for (int i = 0; i < SA; ++i) c.yh[0][i] = c.y[i];
   c.pc = PC_260;
   c.req = REQ_NONE;
   c.outcome = OUT_OK;
@@ -679,13 +845,13 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
   if (wave_any(at(c, PC_NTL_F)) && at(c, PC_NTL_F)) {
     c.nfe++;
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.save1[i] = c.save2[i] / dmax(1.0, c.ywt[i]);
+    for (int i = 0; i < SA; ++i) c.save1[i] = c.save2[i] / dmax(1.0, c.ywt[i]);
     const double sum = dnrm2(c.save1) / k.sqrtn;
     if (sum > k.eps / fabs(c.h)) c.h = copysign(k.eps / sum, c.h);
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.yh[1][i] = c.h * c.save2[i];
+    for (int i = 0; i < SA; ++i) c.yh[1][i] = c.h * c.save2[i];
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.fac[i] = k.sqrt_uround;
+    for (int i = 0; i < SA; ++i) c.fac[i] = k.sqrt_uround;
     if (c.ntl_restart) c.rmax = 10.0;
     if (c.h == 0.0) {
       copy_yh0_to_y(c);
@@ -703,7 +869,7 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
         c.req = REQ_JAC;
       } else {
 #pragma unroll
-        for (int i = 0; i < S; ++i) c.save1[i] = 0.0;
+        for (int i = 0; i < SA; ++i) c.save1[i] = 0.0;
         c.pc = PC_CORR;
       }
     } else if (at(c, PC_JAC_DONE)) {
@@ -714,7 +880,7 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
         c.rc = 1.0;
         c.jstepl = c.nstep;
 #pragma unroll
-        for (int i = 0; i < S; ++i) c.save1[i] = 0.0;
+        for (int i = 0; i < SA; ++i) c.save1[i] = 0.0;
         c.pc = PC_CORR;
       }
     } else if (at(c, PC_CORR_F)) {
@@ -735,7 +901,7 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
       if (c.iter < 3) {
         const double l0 = el_of(mc, 1, c.nq);
 #pragma unroll
-        for (int i = 0; i < S; ++i) c.y[i] = c.yh[0][i] + l0 * c.save1[i];
+        for (int i = 0; i < SA; ++i) c.y[i] = c.yh[0][i] + l0 * c.save1[i];
         request_f(c, PC_CORR_F);
       } else if (c.convrg) {                           // stale Jacobian: refresh (label 110)
         c.evaljc = true;
@@ -748,10 +914,17 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
     }
   }
   // ---- 4: error test, accept, step/order selection, label 170 (:222-432)
-  if (wave_any(at(c, PC_ERRTEST)) && at(c, PC_ERRTEST)) {
+  // A converged lane may wait here (it asks for nothing, so it re-enters on
+  // the next pass unchanged) until gather_pct % of the lanes stepping in this
+  // pass have arrived: the stage's code then runs once for all of them.  The
+  // others always make progress, so the share is reached.
+  const bool here4 = at(c, PC_ERRTEST);
+  const int n4 = __popcll(__ballot(here4 ? 1 : 0));
+  const int nact = __popcll(__ballot(1));
+  if (n4 > 0 && n4 * 100 >= k.gather_pct * nact && here4) {
     c.convrg = true;
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.save2[i] = c.save1[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+    for (int i = 0; i < SA; ++i) c.save2[i] = c.save1[i] / dmax(fabs(c.y[i]), c.ywt[i]);
     const double etest = dnrm2(c.save2) / (tq_of(mc, 2, c.nq) * k.sqrtn);
     c.etest = etest;
     if (etest > k.eps) {
@@ -765,7 +938,7 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
         if (j <= c.nq + 1) {
           const double e = el_of(mc, j, c.nq);
 #pragma unroll
-          for (int i = 0; i < S; ++i) c.yh[j - 1][i] = c.yh[j - 1][i] + e * c.save1[i];
+          for (int i = 0; i < SA; ++i) c.yh[j - 1][i] = c.yh[j - 1][i] + e * c.save1[i];
         }
       }
       copy_yh0_to_y(c);
@@ -781,26 +954,29 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
         c.nwait = c.nwait - 1;
         if (c.nwait == 1 && c.nq < k.maxord) {
 #pragma unroll
-          for (int i = 0; i < S; ++i) c.yh[MAXORD][i] = c.save1[i];
+          for (int i = 0; i < SA; ++i) c.yh[MAXORD][i] = c.save1[i];
         }
       } else {
-        double rh1 = 0.0, rh3 = 0.0;
-        if (c.nq != 1) {
-          double col[S];
-          yh_col(c, c.nq, col);
+        // The three candidates (order down, same, up) are independent: both
+        // norms come from one call and the three powers from another.  A
+        // candidate that does not apply (order 1 cannot go down, MAXORD not
+        // up) is formed from whatever its vector holds and then not used.
+        double dn[SA], up[SA];
+        yh_col(c, c.nq, dn);
 #pragma unroll
-          for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
-          const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
-          rh1 = 1.0 / dmax(k.uround, 1.3 * nl_pow(erdn / k.eps, 1.0 / (double)c.nq));
+        for (int i = 0; i < SA; ++i) {
+          const double w = dmax(fabs(c.y[i]), c.ywt[i]);
+          dn[i] = dn[i] / w;
+          up[i] = (c.save1[i] - c.yh[MAXORD][i]) / w;
         }
-        const double rh2 = 1.0 / dmax(k.uround, 1.2 * nl_pow(etest / k.eps, 1.0 / (double)(c.nq + 1)));
-        if (c.nq != k.maxord) {
-#pragma unroll
-          for (int i = 0; i < S; ++i)
-            c.save2[i] = (c.save1[i] - c.yh[MAXORD][i]) / dmax(fabs(c.y[i]), c.ywt[i]);
-          const double erup = dnrm2(c.save2) / (tq_of(mc, 3, c.nq) * k.sqrtn);
-          rh3 = 1.0 / dmax(k.uround, 1.4 * nl_pow(erup / k.eps, 1.0 / (double)(c.nq + 2)));
-        }
+        const Norm2 nr = dnrm2_pair(dn, up);
+        const double erdn = nr.a / (tq_of(mc, 1, c.nq) * k.sqrtn);
+        const double erup = nr.b / (tq_of(mc, 3, c.nq) * k.sqrtn);
+        const Pow3 pw = nl_pow3(erdn / k.eps, 1.0 / (double)c.nq, etest / k.eps,
+                                1.0 / (double)(c.nq + 1), erup / k.eps, 1.0 / (double)(c.nq + 2));
+        const double rh1 = (c.nq != 1) ? 1.0 / dmax(k.uround, 1.3 * pw.a) : 0.0;
+        const double rh2 = 1.0 / dmax(k.uround, 1.2 * pw.b);
+        const double rh3 = (c.nq != k.maxord) ? 1.0 / dmax(k.uround, 1.4 * pw.c) : 0.0;
         double rh;
         bool change = true;
         if (rh1 > rh2 && rh1 >= rh3) {
@@ -822,13 +998,13 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
             const double e = el_of(mc, c.nq + 1, c.nq);
             const double den = (double)(c.nq + 1);
             const int dst = c.nq + 1;
-            double newc[S];
+            double newc[SA];
 #pragma unroll
-            for (int i = 0; i < S; ++i) newc[i] = (c.save1[i] * e) / den;
+            for (int i = 0; i < SA; ++i) newc[i] = (c.save1[i] * e) / den;
 #pragma unroll
             for (int j = 2; j <= MAXORD; ++j) {
 #pragma unroll
-              for (int i = 0; i < S; ++i) c.yh[j][i] = (dst == j) ? newc[i] : c.yh[j][i];
+              for (int i = 0; i < SA; ++i) c.yh[j][i] = (dst == j) ? newc[i] : c.yh[j][i];
             }
             c.nq = c.nq + 1;
             c.rc = (c.rc * el_of(mc, 1, c.nq)) / el_of(mc, 1, c.nq - 1);
@@ -872,10 +1048,10 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
         const double rh2 = 1.0 / (1.2 * nl_pow(c.etest / k.eps, 1.0 / (double)(c.nq + 1)));
         double rh = rh2;
         if (c.nq > 1) {
-          double col[S];
+          double col[SA];
           yh_col(c, c.nq, col);
 #pragma unroll
-          for (int i = 0; i < S; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
+          for (int i = 0; i < SA; ++i) c.save2[i] = col[i] / dmax(fabs(c.y[i]), c.ywt[i]);
           const double erdn = dnrm2(c.save2) / (tq_of(mc, 1, c.nq) * k.sqrtn);
           const double rh1 = 1.0 / dmax(1.0, 1.3 * nl_pow(erdn / k.eps, 1.0 / (double)c.nq));
           if (rh2 < rh1) {
@@ -923,12 +1099,27 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
   // ---- 6: DDRIV3 label 260: weights and guards, DDSTP entry (slatec/ddriv3.f90:1292-1351)
   if (wave_any(at(c, PC_260)) && at(c, PC_260)) {
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.ywt[i] = dmax(k.ewt, fabs(c.y[i]));
+    for (int i = 0; i < SA; ++i) c.ywt[i] = dmax(k.ewt, fabs(c.y[i]));
+    // |y_i / YWT_i| <= 1, so SUM = max(1, ||y/YWT|| / sqrt(N)) stays below 2
+    // and EPS < SUM * UROUND cannot hold for EPS >= 2 * UROUND; the norm is
+    // formed only when it can decide (its SAVE2 is dead: the next f overwrites it).
+    bool eps_fail = false;
+    if (k.eps < 2.0 * k.uround) {
+      double v[S];
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.save2[i] = c.y[i] / c.ywt[i];
-    double sum = dnrm2(c.save2) / k.sqrtn;
-    sum = dmax(1.0, sum);
-    if (k.eps < sum * k.uround) {
+      for (int sp = 0; sp < S; ++sp) {
+        if (kSp.act[sp] >= 0) {
+          v[sp] = c.y[kSp.act[sp]] / c.ywt[kSp.act[sp]];
+        } else {
+          const double yv = c.yi[kSp.islot[sp]];
+          v[sp] = yv / dmax(k.ewt, fabs(yv));
+        }
+      }
+      double sum = dnrm2_all(v) / k.sqrtn;
+      sum = dmax(1.0, sum);
+      eps_fail = k.eps < sum * k.uround;
+    }
+    if (eps_fail) {
       copy_yh0_to_y(c);
       finish(c, OUT_EPS);
     } else if (c.nstep == k.mxstep) {
@@ -980,10 +1171,10 @@ NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc,
   }
   const double l0 = mine ? el_of(mc, 1, c.nq) : 0.0;
 #pragma unroll
-  for (int j = 0; j < S; ++j) {
+  for (int j = 0; j < SA; ++j) {
     double ywj = 0.0, yj = 0.0, fj = 0.0, s2j = 0.0, yh2j = 0.0;
 #pragma unroll
-    for (int q = 0; q < S; ++q) {
+    for (int q = 0; q < SA; ++q) {
       const bool hit = (q == j);
       ywj = hit ? opaque(c.ywt[q]) : ywj;
       yj = hit ? opaque(c.y[q]) : yj;
@@ -991,7 +1182,7 @@ NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc,
       s2j = hit ? opaque(c.save2[q]) : s2j;
       yh2j = hit ? opaque(c.yh[2][q]) : yh2j;
     }
-    double f1[S];
+    double f1[SA];
     if (mine) {
       const double ys = dmax(fabs(ywj), fabs(yj));
       double dy = fj * ys;
@@ -1008,18 +1199,18 @@ NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc,
       dy = (yj + dy) - yj;
       const double yp = yj + dy;
 #pragma unroll
-      for (int q = 0; q < S; ++q) c.y[q] = (q == j) ? yp : c.y[q];
+      for (int q = 0; q < SA; ++q) c.y[q] = (q == j) ? yp : c.y[q];
       rhs(kl, Mc, c.y, f1);
 #pragma unroll
-      for (int q = 0; q < S; ++q) c.y[q] = (q == j) ? yj : c.y[q];
+      for (int q = 0; q < SA; ++q) c.y[q] = (q == j) ? yj : c.y[q];
       const double factor = ((-l0) * c.h) / dy;
-      double col[S];
+      double col[SA];
 #pragma unroll
-      for (int i = 0; i < S; ++i) col[i] = (f1[i] - c.save2[i]) * factor;
+      for (int i = 0; i < SA; ++i) col[i] = (f1[i] - c.save2[i]) * factor;
       double diff = fabs(c.save2[0] - f1[0]);
       double s2m = c.save2[0], s1m = f1[0];
 #pragma unroll
-      for (int i = 1; i < S; ++i) {
+      for (int i = 1; i < SA; ++i) {
         const double di = fabs(c.save2[i] - f1[i]);
         if (di > diff) {
           diff = di;
@@ -1037,18 +1228,18 @@ NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc,
           fj = dmin(k.bp * fj, 0.5);
       }
 #pragma unroll
-      for (int q = 0; q < S; ++q) {
+      for (int q = 0; q < SA; ++q) {
         const bool hit = (q == j);
         c.fac[q] = hit ? fj : c.fac[q];
 #pragma unroll
-        for (int i = 0; i < S; ++i) c.P[q][i] = hit ? col[i] : c.P[q][i];
+        for (int i = 0; i < SA; ++i) c.P[q][i] = hit ? col[i] : c.P[q][i];
       }
     }
   }
   if (mine) {
-    c.nfe += S;
+    c.nfe += S;                                        // DDPST counts N evaluations
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.P[i][i] = c.P[i][i] + 1.0;
+    for (int i = 0; i < SA; ++i) c.P[i][i] = c.P[i][i] + 1.0;
   }
   // DGEFA for the lanes of the batch (the pivot ballots inside see only them)
   if (mine) {
@@ -1081,9 +1272,9 @@ NRG_DI void jac_analytic(Cell &c, bool mine, const MechCoef &mc, const double *k
   c.ier = false;
   const double fac = (-el_of(mc, 1, c.nq)) * c.h;
 #pragma unroll
-  for (int q = 0; q < S; ++q)
+  for (int q = 0; q < SA; ++q)
 #pragma unroll
-    for (int i = 0; i < S; ++i) c.P[q][i] = 0.0;
+    for (int i = 0; i < SA; ++i) c.P[q][i] = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const double m = Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0];
@@ -1100,19 +1291,19 @@ NRG_DI void jac_analytic(Cell &c, bool mine, const MechCoef &mc, const double *k
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
           const int sb = side == 0 ? topo::left[r][b] : topo::right[r][b];
-          if (b < n && b != a) d = d * (sb == S ? m : c.y[sb < S ? sb : 0]);
+          if (b < n && b != a) d = d * (sb == S ? m : c.y[act_of(sb)]);
         }
         const double fd = fac * d;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          const int nu = nu_of(r, s);
-          if (nu != 0) c.P[sa][s] = c.P[sa][s] + (double)nu * fd;
+        for (int i = 0; i < SA; ++i) {
+          const int nu = nu_of(r, kSp.full[i]);
+          if (nu != 0) c.P[act_of(sa)][i] = c.P[act_of(sa)][i] + (double)nu * fd;
         }
       }
     }
   }
 #pragma unroll
-  for (int i = 0; i < S; ++i) c.P[i][i] = c.P[i][i] + 1.0;
+  for (int i = 0; i < SA; ++i) c.P[i][i] = c.P[i][i] + 1.0;
   if (gefa(c.P, c.ipvt) != 0) c.ier = true;
 }
 

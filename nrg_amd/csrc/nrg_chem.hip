@@ -125,8 +125,7 @@ __global__ void reset_kernel(unsigned long long *acc, int n)
 // Load one cell and set it up (chemical_kinetics_solver.f90:228-265).
 // Returns false for a gated cell, whose outputs are written here (:341-346).
 NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long t, long long q,
-                       long long &off, Cell &c, double *kl, double (&Mc)[NM],
-                       unsigned long long (&cnt)[8])
+                       long long &off, Cell &c, double *kl, double (&Mc)[NM])
 {
   off = cell_offset(a.g, q);
   const double T = a.T[off];
@@ -142,12 +141,17 @@ NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long t, long 
       a.nje[q] = 0;
     }
     if (a.hist) a.hist[q] = 0;
-    cnt[NRG_CELL_GATED]++;
     return false;
   }
   const double rho = a.rho[off];
 #pragma unroll
-  for (int s = 0; s < S; ++s) c.y[s] = (a.Y[s][off] * rho) / mc.W[s];
+  for (int s = 0; s < S; ++s) {
+    const double cs = (a.Y[s][off] * rho) / mc.W[s];
+    if (kSp.act[s] >= 0)
+      c.y[kSp.act[s]] = cs;
+    else
+      c.yi[kSp.islot[s]] = cs;
+  }
   const double *rk = a.rates + t;
 #pragma unroll
   for (int r = 0; r < 2 * R; ++r) kl[r * WAVE] = rk[r * a.nchunk];
@@ -178,17 +182,18 @@ __global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
   rate_constants<InlineMath>(mc, T, C, a.rates_out + t, a.nchunk);
 }
 
-// Output assembly (:318-340) once the cell's DDRIV3 has returned.
-NRG_DI void finish_cell(const StepArgs &a, const MechCoef &mc, long long q, long long off,
-                        const Cell &c, unsigned long long (&cnt)[8], unsigned long long (&work)[6],
-                        int &mx_step, int &mx_fe)
+// Output assembly (:318-340) once the cell's DDRIV3 has returned; returns the
+// cell's NRG_CELL_* status.
+NRG_DI int finish_cell(const StepArgs &a, const MechCoef &mc, long long q, long long off,
+                       const Cell &c)
 {
   const double rho = a.rho[off];
   double ef = 0.0;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const double cin = (a.Y[s][off] * rho) / mc.W[s];
-    const double dc = c.y[s] - cin;
+    const double ys = (kSp.act[s] >= 0) ? c.y[kSp.act[s]] : c.yi[kSp.islot[s]];
+    const double dc = ys - cin;
     a.Yp[s][off] = (dc * mc.W[s]) / a.dt;
     a.Yp2[s][off] = dc;
     ef = ef - (mc.h_ref[s] * dc) / a.dt;
@@ -206,17 +211,49 @@ NRG_DI void finish_cell(const StepArgs &a, const MechCoef &mc, long long q, long
     a.nje[q] = c.nje;
   }
   if (a.hist) a.hist[q] = c.nstep;
-#pragma unroll
-  for (int st = 0; st < 8; ++st) cnt[st] += (status == st) ? 1ull : 0ull;
-  work[0] += 1;
-  work[1] += c.nstep;
-  work[2] += c.nfe;
-  work[3] += c.nje;
-  work[4] += c.ncorr;
-  work[5] += c.natt;
-  mx_step = max(mx_step, c.nstep);
-  mx_fe = max(mx_fe, c.nfe);
+  return status;
 }
+
+// Per-lane work and status counts, reduced once per wave at the end.
+struct Tally {
+  unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long work[6] = {0, 0, 0, 0, 0, 0};
+  int mx_step = 0, mx_fe = 0;
+
+  NRG_DI void integrated(const Cell &c, int status)
+  {
+#pragma unroll
+    for (int st = 0; st < 8; ++st) cnt[st] += (status == st) ? 1ull : 0ull;
+    work[0] += 1;
+    work[1] += c.nstep;
+    work[2] += c.nfe;
+    work[3] += c.nje;
+    work[4] += c.ncorr;
+    work[5] += c.natt;
+    mx_step = max(mx_step, c.nstep);
+    mx_fe = max(mx_fe, c.nfe);
+  }
+
+  NRG_DI void flush(unsigned long long *acc, int lane)
+  {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const unsigned long long w = wave_sum_u64(work[i]);
+      if (lane == 0 && w) atomicAdd(&acc[ACC_INTEG + i], w);
+    }
+    const int ms = wave_max_i32(mx_step);
+    const int mf = wave_max_i32(mx_fe);
+    if (lane == 0) {
+      atomicMax(&acc[ACC_NSTEP_MAX], (unsigned long long)ms);
+      atomicMax(&acc[ACC_NFE_MAX], (unsigned long long)mf);
+    }
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+      const unsigned long long w = wave_sum_u64(cnt[st]);
+      if (lane == 0 && w) atomicAdd(&acc[ACC_STATUS0 + st], w);
+    }
+  }
+};
 
 // One chemistry step (solve_chemical_kinetics, :224-346), one cell per lane.
 // PERSISTENT = false: lane i of block b owns inner cell 64*b + i (the plain
@@ -244,14 +281,18 @@ chem_bdf_kernel(StepArgs a)
   c.req = REQ_DONE;
   long long q = -1, off = 0;
   bool have = false, exhausted = false;
-  unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  unsigned long long work[6] = {0, 0, 0, 0, 0, 0};
-  int mx_step = 0, mx_fe = 0;
+  // The persistent form tallies each finished cell as it goes; a lane of the
+  // plain form owns one cell, whose counts are read from its state at the end
+  // (keeping the tally out of the registers live across the loop).
+  Tally tl;
+  bool gated = false;
+  int status = 0;
   if (!PERSISTENT) {
     const long long t = (long long)blockIdx.x * WAVE + lane;
     if (t < a.nchunk) {
       q = cell_of(a, t);
-      have = start_cell(a, mc, t, q, off, c, kl, Mc, cnt);
+      have = start_cell(a, mc, t, q, off, c, kl, Mc);
+      gated = !have;
     }
   }
   for (;;) {
@@ -270,7 +311,8 @@ chem_bdf_kernel(StepArgs a)
             exhausted = true;
           } else {
             q = cell_of(a, t);
-            have = start_cell(a, mc, t, q, off, c, kl, Mc, cnt);
+            have = start_cell(a, mc, t, q, off, c, kl, Mc);
+            if (!have) tl.cnt[NRG_CELL_GATED]++;
           }
         }
         need = !have && !exhausted;
@@ -295,28 +337,19 @@ chem_bdf_kernel(StepArgs a)
       advance(c, a.dt, a.k, mc);
     }
     if (have && c.req == REQ_DONE) {
-      finish_cell(a, mc, q, off, c, cnt, work, mx_step, mx_fe);
+      status = finish_cell(a, mc, q, off, c);
+      if (PERSISTENT) tl.integrated(c, status);
       have = false;
     }
     if (!PERSISTENT && !wave_any(have)) break;
   }
-  // work counters: one atomic per wave and slot
-#pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    const unsigned long long w = wave_sum_u64(work[i]);
-    if (lane == 0 && w) atomicAdd(&a.acc[ACC_INTEG + i], w);
+  if (!PERSISTENT) {
+    if (gated)
+      tl.cnt[NRG_CELL_GATED]++;
+    else if (q >= 0)
+      tl.integrated(c, status);
   }
-  mx_step = wave_max_i32(mx_step);
-  mx_fe = wave_max_i32(mx_fe);
-  if (lane == 0) {
-    atomicMax(&a.acc[ACC_NSTEP_MAX], (unsigned long long)mx_step);
-    atomicMax(&a.acc[ACC_NFE_MAX], (unsigned long long)mx_fe);
-  }
-#pragma unroll
-  for (int st = 0; st < 6; ++st) {
-    const unsigned long long w = wave_sum_u64(cnt[st]);
-    if (lane == 0 && w) atomicAdd(&a.acc[ACC_STATUS0 + st], w);
-  }
+  tl.flush(a.acc, lane);   // one atomic per wave and slot
 }
 
 // ---------------------------------------------------------------------------
@@ -941,6 +974,7 @@ BdfConst make_const(const nrg_chem_opts &o)
   k.t_gate = o.t_gate;
   k.mxstep = o.mxstep;
   k.maxord = std::min(o.mxord, MAXORD);
+  k.gather_pct = o.errtest_gather_pct;
   return k;
 }
 
@@ -1038,6 +1072,8 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     nrg_chem_default_opts(&o);
   if (!(o.eps > 0.0) || !(o.ewt > 0.0) || !(o.hmax > 0.0) || o.mxstep <= 0 || o.mxord <= 0)
     return set_error(NRG_CHEM_E_ARG, "invalid integrator options");
+  if (o.errtest_gather_pct < 0 || o.errtest_gather_pct > 100)
+    return set_error(NRG_CHEM_E_ARG, "errtest_gather_pct must be in 0..100");
   if (o.jacobian != NRG_JAC_FD_SLATEC && o.jacobian != NRG_JAC_ANALYTIC)
     return set_error(NRG_CHEM_E_ARG, "jacobian mode %d not available", o.jacobian);
   if (o.form != NRG_FORM_PLAIN && o.form != NRG_FORM_PERSISTENT)
@@ -1549,8 +1585,10 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
   struct Cfg {
     int form, order, jac, wpc;
     long long chunk;
+    int gather;
   };
   auto apply = [&](const Cfg &c) {
+    h->k.gather_pct = c.gather;
     h->form = c.form;
     h->order = c.order;
     h->jac_pct = c.jac;
@@ -1567,6 +1605,7 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
       nrg_chem_tune_trial &t = out->trials[out->n_trials++];
       t.form = c.form; t.cell_order = c.order; t.jac_batch_pct = c.jac;
       t.waves_per_cu = c.wpc > 0 ? c.wpc : occ; t.chunk_cells = c.chunk; t.kernel_ms = ms;
+      t.errtest_gather_pct = c.gather;
     }
     return st;
   };
@@ -1579,7 +1618,7 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
     ~Restore() { h->use_graph = g; }
   } restore{h, graph};
   const int wpc0 = h->persistent_blocks / std::max(prop.multiProcessorCount, 1);
-  Cfg best{h->form, h->order, h->jac_pct, wpc0 == occ ? 0 : wpc0, h->chunk_max};
+  Cfg best{h->form, h->order, h->jac_pct, wpc0 == occ ? 0 : wpc0, h->chunk_max, h->k.gather_pct};
   double best_ms = 0.0;
   int st;
   // the first run sets up the stiffness history that the ordered trials use
@@ -1607,6 +1646,12 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
       c.form = form;
       if ((st = consider(c))) return st;
     }
+  for (int gather : {0, 100})
+    if (gather != best.gather) {
+      Cfg c = best;
+      c.gather = gather;
+      if ((st = consider(c))) return st;
+    }
   for (int jac : {25, 50, 75, 100})
     if (jac != best.jac) {
       Cfg c = best;
@@ -1632,6 +1677,7 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
   out->best.waves_per_cu = best.wpc > 0 ? best.wpc : occ;
   out->best.chunk_cells = best.chunk;
   out->best.kernel_ms = best_ms;
+  out->best.errtest_gather_pct = best.gather;
   return NRG_CHEM_OK;
 }
 

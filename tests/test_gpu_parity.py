@@ -200,6 +200,27 @@ def test_stiffness_order_is_bitwise_identical(mech, engine, form):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("form,gather", [(capi.NRG_FORM_PLAIN, 100), (capi.NRG_FORM_PLAIN, 50),
+                                         (capi.NRG_FORM_PERSISTENT, 100)])
+def test_errtest_gather_is_bitwise_identical(mech, engine, form, gather):
+    """errtest_gather_pct only delays converged lanes at the error test until
+    enough of the wave is there: every cell takes the same steps, so outputs
+    and work counters equal the ungathered step bit for bit."""
+    st = _to_cuda(inputs.ignition_mix(mech, 20000, seed=37, frac_window=0.3))
+    from nrg_amd.solver import ChemistryStep
+    eng = ChemistryStep(mech, capi.default_opts(form=form, errtest_gather_pct=gather))
+    try:
+        for dt in (1e-7, 1e-6):
+            a = gpu_flat_step(engine, st, dt)
+            b = gpu_flat_step(eng, st, dt)
+            _report("errtest_gather", form=form, gather=gather, dt=dt, nfe=b["stats"].nfe_sum)
+            for k in ("Y_prod", "Y_prod2", "E_f_prod", "status", "nstep", "nfe", "nje"):
+                assert np.array_equal(a[k], b[k]), (dt, k)
+            assert a["stats"].ncorr_sum == b["stats"].ncorr_sum
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("order", [capi.NRG_ORDER_NATURAL, capi.NRG_ORDER_STIFFNESS])
 def test_graph_replay_is_bitwise_identical(mech, engine, order):
     """use_graph = 1: the persistent step is captured once per argument set and
