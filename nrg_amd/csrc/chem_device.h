@@ -60,6 +60,14 @@ struct MechCoef {
   double patm_r;                 // P_atm / R_u
 };
 
+// The part of the mechanism the integrator kernel reads (one LDS copy per block).
+struct BdfCoef {
+  double el[MAXORD][MAXORD + 1];
+  double tq[MAXORD][3];
+  double W[S];
+  double h_ref[S];
+};
+
 // Integrator constants of one call (reference literals + D1MACH-derived values).
 struct BdfConst {
   double eps, ewt, hmax, uround, sqrtn, sqrt_uround;
@@ -88,9 +96,24 @@ __device__ __noinline__ Pow3 nl_pow3(double xa, double ya, double xb, double yb,
 }
 
 // Math for the rate-constant kernel, where inlining is the better trade.
+// Cases whose IEEE result is exact skip the expansion: pow(x, +-0) = 1,
+// pow(x, 1) = x, exp(+-0) = 1; exp(x) rounds to 1 for |x| < 2^-54 and to +0
+// below -746 (under half the least subnormal).  The mechanism's zero
+// temperature exponents and activation energies and its disabled Troe terms
+// (T*** = 0, T* = T** = 1e30) hit them for every cell alike.
 struct InlineMath {
-  static NRG_DI double pow_(double x, double y) { return pow(x, y); }
-  static NRG_DI double exp_(double x) { return exp(x); }
+  static NRG_DI double pow_(double x, double y)
+  {
+    if (y == 0.0) return 1.0;
+    if (y == 1.0) return x;
+    return pow(x, y);
+  }
+  static NRG_DI double exp_(double x)
+  {
+    if (fabs(x) < 0x1p-60) return 1.0;
+    if (x < -746.0) return 0.0;
+    return exp(x);
+  }
   static NRG_DI double log_(double x) { return log(x); }
   static NRG_DI double log10_(double x) { return log10(x); }
 };
@@ -387,6 +410,12 @@ NRG_DI double dnrm2_core(const double *x)
 #pragma unroll
   for (int i = 0; i < n; ++i) {
     const double v = x[i], av = fabs(v);
+    // the usual case first (phase 3, no overflow risk): what the state
+    // machine below does for it, without walking through the other phases
+    if (st == 4 && av < hitest) {
+      sum = sum + v * v;
+      continue;
+    }
     bool done = false;
     if (st == 0) {
       if (av > cutlo) {
@@ -674,8 +703,8 @@ struct Cell {
   bool fail_err, do_scale, rmax_after, after_accept;
 };
 
-NRG_DI double el_of(const MechCoef &mc, int j, int nq) { return mc.el[nq - 1][j - 1]; }
-NRG_DI double tq_of(const MechCoef &mc, int k, int nq) { return mc.tq[nq - 1][k - 1]; }
+NRG_DI double el_of(const BdfCoef &mc, int j, int nq) { return mc.el[nq - 1][j - 1]; }
+NRG_DI double tq_of(const BdfCoef &mc, int k, int nq) { return mc.tq[nq - 1][k - 1]; }
 
 // DDPSC (slatec/ddpsc.f90:26-38) for a compile-time order.
 template <int NQ, bool ADD>
@@ -839,7 +868,7 @@ NRG_DI bool at(const Cell &c, int pc) { return c.req == REQ_NONE && c.pc == pc; 
 //   4 error test, accept, step/order choice (label 170)
 //   5 corrector failure (label 160)   6 DDRIV3 label 260   7 attempt (label 100)
 // Each stage's code exists once; a lane may pass several stages per call.
-NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
+NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const BdfCoef &mc)
 {
   // ---- 1: rest of DDNTL after its f(y) (slatec/ddntl.f90:68,152-161)
   if (wave_any(at(c, PC_NTL_F)) && at(c, PC_NTL_F)) {
@@ -1162,7 +1191,7 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const MechCoef &mc)
 // into P are plain register moves instead of select chains over all nine
 // columns (measured: 11-18 % less time per step than one shared loop body,
 // despite nine RHS copies).
-NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const MechCoef &mc, const double *kl,
+NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const BdfCoef &mc, const double *kl,
                       const double (&Mc)[NM])
 {
   if (mine) {
@@ -1264,7 +1293,7 @@ constexpr int nu_of(int r, int s)
 // has no derivative), as DDPST forms it for a user Jacobian (MITER = 1).  No
 // extra right-hand sides are evaluated; the step sequence therefore differs
 // from the finite-difference (SLATEC MITER = 2) parity mode.
-NRG_DI void jac_analytic(Cell &c, bool mine, const MechCoef &mc, const double *kl,
+NRG_DI void jac_analytic(Cell &c, bool mine, const BdfCoef &mc, const double *kl,
                          const double (&Mc)[NM])
 {
   if (!mine) return;

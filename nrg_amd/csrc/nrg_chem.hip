@@ -94,6 +94,18 @@ NRG_DI void stage_mech(MechCoef &dst, const MechCoef *src)
   __syncthreads();
 }
 
+NRG_DI void stage_bdf(BdfCoef &dst, const MechCoef *src)
+{
+  for (int i = threadIdx.x; i < MAXORD * (MAXORD + 1); i += blockDim.x)
+    (&dst.el[0][0])[i] = (&src->el[0][0])[i];
+  for (int i = threadIdx.x; i < MAXORD * 3; i += blockDim.x) (&dst.tq[0][0])[i] = (&src->tq[0][0])[i];
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    dst.W[i] = src->W[i];
+    dst.h_ref[i] = src->h_ref[i];
+  }
+  __syncthreads();
+}
+
 NRG_DI unsigned long long wave_sum_u64(unsigned long long v)
 {
 #pragma unroll
@@ -124,7 +136,7 @@ __global__ void reset_kernel(unsigned long long *acc, int n)
 
 // Load one cell and set it up (chemical_kinetics_solver.f90:228-265).
 // Returns false for a gated cell, whose outputs are written here (:341-346).
-NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long t, long long q,
+NRG_DI bool start_cell(const StepArgs &a, const BdfCoef &mc, long long t, long long q,
                        long long &off, Cell &c, double *kl, double (&Mc)[NM])
 {
   off = cell_offset(a.g, q);
@@ -163,8 +175,10 @@ NRG_DI bool start_cell(const StepArgs &a, const MechCoef &mc, long long t, long 
 
 // Rate constants and [M] of every integrated cell of the chunk, one cell per
 // thread (chemical_kinetics_solver.f90:235-252, 360-493).  Kept apart from the
-// integrator so its ~120 transcendentals run coherently at full occupancy.
-__global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
+// integrator so its ~80 transcendentals run coherently at two waves per SIMD
+// (the register budget that allows costs a few spilled values; one wave per
+// SIMD, the unconstrained allocation, measured 3 ms slower per 2^24 cells).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) rates_kernel(StepArgs a)
 {
   __shared__ MechCoef mc;
   stage_mech(mc, a.mech);
@@ -184,7 +198,7 @@ __global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
 
 // Output assembly (:318-340) once the cell's DDRIV3 has returned; returns the
 // cell's NRG_CELL_* status.
-NRG_DI int finish_cell(const StepArgs &a, const MechCoef &mc, long long q, long long off,
+NRG_DI int finish_cell(const StepArgs &a, const BdfCoef &mc, long long q, long long off,
                        const Cell &c)
 {
   const double rho = a.rho[off];
@@ -269,9 +283,9 @@ template <bool PERSISTENT, bool ANALYTIC>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(NRG_BDF_WAVES_PER_EU)))
 chem_bdf_kernel(StepArgs a)
 {
-  __shared__ MechCoef mc;
+  __shared__ BdfCoef mc;
   __shared__ double krate[2 * R * WAVE];
-  stage_mech(mc, a.mech);
+  stage_bdf(mc, a.mech);
   const int lane = threadIdx.x;
   double *kl = krate + lane;
   Cell c;
@@ -484,9 +498,13 @@ __global__ void __launch_bounds__(256) energy_kernel(EnergyArgs a)
 
 // ---------------------------------------------------------------------------
 // Stiffness-ordered queue: bin the inner cells by expected cost, most
-// expensive bin first; within a block the cells keep their relative order.
+// expensive bin first.  A block bins a contiguous run of BIN_RUN cells (one
+// global atomic per bin and block); within a bin, the cells of a run come in
+// roughly increasing index order.
 // ---------------------------------------------------------------------------
 constexpr int NBIN = 64;
+constexpr int BIN_THREADS = 256;
+constexpr int BIN_RUN = 32 * BIN_THREADS;
 
 struct BinArgs {
   GridIdx g;
@@ -515,14 +533,23 @@ NRG_DI int cost_bin(const BinArgs &a, long long q)
   return 1 + (x < (double)(NBIN - 2) ? (int)x : NBIN - 2);
 }
 
-__global__ void __launch_bounds__(256) bin_count_kernel(BinArgs a)
+// Histogram of the block's run into LDS.
+NRG_DI void bin_histogram(const BinArgs &a, unsigned int *h)
 {
-  __shared__ unsigned int h[NBIN];
   if (threadIdx.x < NBIN) h[threadIdx.x] = 0u;
   __syncthreads();
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < a.g.ninner) atomicAdd(&h[cost_bin(a, q)], 1u);
+  const long long q0 = (long long)blockIdx.x * BIN_RUN + threadIdx.x;
+  for (int k = 0; k < BIN_RUN / BIN_THREADS; ++k) {
+    const long long q = q0 + (long long)k * BIN_THREADS;
+    if (q < a.g.ninner) atomicAdd(&h[cost_bin(a, q)], 1u);
+  }
   __syncthreads();
+}
+
+__global__ void __launch_bounds__(BIN_THREADS) bin_count_kernel(BinArgs a)
+{
+  __shared__ unsigned int h[NBIN];
+  bin_histogram(a, h);
   if (threadIdx.x < NBIN && h[threadIdx.x]) atomicAdd(&a.count[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
@@ -538,21 +565,27 @@ __global__ void bin_offsets_kernel(BinArgs a)
   }
 }
 
-__global__ void __launch_bounds__(256) bin_scatter_kernel(BinArgs a)
+// Each block claims its bins' slots with one atomic per bin, then places its
+// run's cells (same run as bin_count_kernel, binned again).
+__global__ void __launch_bounds__(BIN_THREADS) bin_scatter_kernel(BinArgs a)
 {
   __shared__ unsigned int h[NBIN];
   __shared__ unsigned long long base[NBIN];
-  if (threadIdx.x < NBIN) h[threadIdx.x] = 0u;
+  bin_histogram(a, h);
+  if (threadIdx.x < NBIN) {
+    base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&a.cursor[threadIdx.x], (unsigned long long)h[threadIdx.x]) : 0ull;
+    h[threadIdx.x] = 0u;
+  }
   __syncthreads();
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = q < a.g.ninner ? cost_bin(a, q) : -1;
-  unsigned int rank = 0;
-  if (b >= 0) rank = atomicAdd(&h[b], 1u);
-  __syncthreads();
-  if (threadIdx.x < NBIN && h[threadIdx.x])
-    base[threadIdx.x] = atomicAdd(&a.cursor[threadIdx.x], (unsigned long long)h[threadIdx.x]);
-  __syncthreads();
-  if (b >= 0) a.list[base[b] + rank] = (int32_t)q;
+  const long long q0 = (long long)blockIdx.x * BIN_RUN + threadIdx.x;
+  for (int k = 0; k < BIN_RUN / BIN_THREADS; ++k) {
+    const long long q = q0 + (long long)k * BIN_THREADS;
+    if (q < a.g.ninner) {
+      const int b = cost_bin(a, q);
+      const unsigned int rank = atomicAdd(&h[b], 1u);
+      a.list[base[b] + rank] = (int32_t)q;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1193,11 +1226,11 @@ static void enqueue_order(nrg_chem_t *h, StepArgs &a, bool use_hist)
   b.count = h->bins;
   b.cursor = h->bins + NBIN;
   b.list = h->order_list;
-  const unsigned nb = (unsigned)((n + 255) / 256);
+  const unsigned nb = (unsigned)((n + BIN_RUN - 1) / BIN_RUN);
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(2 * NBIN), 0, h->stream, h->bins, 2 * NBIN);
-  hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, h->stream, b);
+  hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(BIN_THREADS), 0, h->stream, b);
   hipLaunchKernelGGL(bin_offsets_kernel, dim3(1), dim3(64), 0, h->stream, b);
-  hipLaunchKernelGGL(bin_scatter_kernel, dim3(nb), dim3(256), 0, h->stream, b);
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(nb), dim3(BIN_THREADS), 0, h->stream, b);
 }
 
 // Enqueue rates_kernel + chem_bdf_kernel over npos queue positions (all inner
