@@ -35,22 +35,22 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c)
 
 // GELU, tanh form (Hendrycks & Gimpel 2016):
 //   g(z) = z/2 (1 + tanh(u)),  u = sqrt(2/pi) (z + 0.044715 z^3)
-__device__ __forceinline__ float tanh_f(float u)
+// evaluated as z * sigma(2u) (1/2 (1 + tanh(u)) = 1 / (1 + e^(-2u))): one
+// exp2 and one reciprocal, with -2 sqrt(2/pi) log2(e) folded into the cubic.
+__device__ __forceinline__ float sigma2u(float z)
 {
-  return 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
+  const float a0 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+  const float a1 = a0 * 0.044715f;
+  const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(z * z, a1, a0));
+  return __builtin_amdgcn_rcpf(1.0f + e);
 }
-__device__ __forceinline__ float gelu(float z)
-{
-  const float c = 0.7978845608028654f, a = 0.044715f;
-  const float t = tanh_f(c * (z + a * z * z * z));
-  return 0.5f * z * (1.0f + t);
-}
+__device__ __forceinline__ float gelu(float z) { return z * sigma2u(z); }
 __device__ __forceinline__ float gelu_grad(float z)
 {
   const float c = 0.7978845608028654f, a = 0.044715f;
-  const float t = tanh_f(c * (z + a * z * z * z));
+  const float sg = sigma2u(z);
   const float dudz = c * (1.0f + 3.0f * a * z * z);
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * dudz;
+  return sg + 2.0f * z * sg * (1.0f - sg) * dudz;
 }
 
 // ---- training GEMM: one wave per 32x32 tile of C, optional split over K ----------
@@ -394,6 +394,142 @@ __global__ void __launch_bounds__(256) mlp_infer2_kernel(InferArgs a)
   }
 }
 
+// ---- inference with the activations in registers ---------------------------------
+// Each layer is computed transposed, Z^T = W^T H^T: the weights are the MFMA A
+// operand (32 output features x 16 inputs) and the activations the B operand
+// (16 inputs x 32 cells).  A lane's accumulator then holds, for its cell, the
+// output features 32 mt + 8 (r/4) + 4 hi + r%4 (r = 0..15, hi = lane/32): after
+// bias and GELU, elements r = 0..7 and 8..15 are exactly the lane's B fragments
+// of k-blocks 2 mt and 2 mt + 1 of the next layer, provided that layer's
+// weights are stored with their K rows in that order (swizzle_t_kernel).  So
+// hidden activations never leave the registers; the hidden layers' weights
+// (H x H bf16, 128 KB at H = 256) are staged in LDS once per layer and block.
+constexpr int TCELLS = 64;          // cells per wave: two 32-cell tiles share each A fragment
+
+// physical feature (output column of the previous layer) of logical K row kl
+__host__ __device__ inline int t_phys(int kl)
+{
+  const int kb = kl / 16, hi = (kl / 8) % 2, v = kl % 8;
+  const int r = 8 * (kb % 2) + v;
+  return 32 * (kb / 2) + 8 * (r / 4) + 4 * hi + r % 4;
+}
+
+// WswT[((mt*KB + kb)*64 + lane)*8 + v] = W[k][n], n = 32 mt + lane%32,
+// k = (permuted ? t_phys : identity)(16 kb + 8 (lane/32) + v).
+__global__ void swizzle_t_kernel(const bf16 *W, int K, int N, int permuted, bf16 *WswT)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)K * N) return;
+  const int v = (int)(i % 8);
+  const int lane = (int)((i / 8) % WAVE);
+  const long long f = i / (8 * WAVE);
+  const int KB = K / 16;
+  const int kb = (int)(f % KB), mt = (int)(f / KB);
+  const int kl = 16 * kb + 8 * (lane / 32) + v;
+  const int k = permuted ? t_phys(kl) : kl;
+  WswT[i] = W[(long long)k * N + 32 * mt + lane % 32];
+}
+
+// bias + GELU of one 32-feature tile, packed as the next layer's two B fragments
+__device__ __forceinline__ void t_epilogue(const f32x16 &acc, const float *bias, int mt, int hi,
+                                           bf16x8 &lo, bf16x8 &up)
+{
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const float z = acc[r] + bias[32 * mt + 8 * (r / 4) + 4 * hi + r % 4];
+    const bf16 g = (bf16)gelu(z);
+    if (r < 8)
+      lo[r] = g;
+    else
+      up[r - 8] = g;
+  }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) mlp_infer_t_kernel(InferArgs a)
+{
+  constexpr int KB = 2 * NT;                      // k-blocks of a hidden layer
+  __shared__ bf16x8 wl[NT * KB * WAVE];           // one hidden layer's A fragments
+  __shared__ float bl[32 * NT];                   // its bias
+  const int w = threadIdx.x / WAVE, l = threadIdx.x & (WAVE - 1);
+  const int r32 = l & 31, hi = l >> 5;
+  const long long cell0 = (long long)blockIdx.x * (4 * TCELLS) + w * TCELLS;
+
+  bf16x8 h[KB][2];
+  // layer 0 (K = 32): input B fragments straight from X, weights from L2
+  {
+    bf16x8 x[2][2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const long long c = cell0 + 32 * ct + r32 < a.n ? cell0 + 32 * ct + r32 : a.n - 1;
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb)
+        x[kb][ct] = *reinterpret_cast<const bf16x8 *>(a.X + c * MLP_IN_PAD + 16 * kb + 8 * hi);
+    }
+    const bf16x8 *W0 = reinterpret_cast<const bf16x8 *>(a.Wsw[0]);
+    const float *b0 = a.bias[0];
+#pragma unroll
+    for (int mt = 0; mt < NT; ++mt) {
+      f32x16 acc[2] = {f32x16{}, f32x16{}};
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        const bf16x8 A = W0[(mt * 2 + kb) * WAVE + l];
+        acc[0] = mfma(A, x[kb][0], acc[0]);
+        acc[1] = mfma(A, x[kb][1], acc[1]);
+      }
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) t_epilogue(acc[ct], b0, mt, hi, h[2 * mt][ct], h[2 * mt + 1][ct]);
+    }
+  }
+  // hidden layers
+  for (int layer = 1; layer < a.layers; ++layer) {
+    __syncthreads();                               // previous layer's weights no longer read
+    const bf16x8 *W = reinterpret_cast<const bf16x8 *>(a.Wsw[layer]);
+#pragma unroll 4
+    for (int i = threadIdx.x; i < NT * KB * WAVE; i += 256) wl[i] = W[i];
+    if (threadIdx.x < 32 * NT) bl[threadIdx.x] = a.bias[layer][threadIdx.x];
+    __syncthreads();
+    bf16x8 hn[KB][2];
+#pragma unroll
+    for (int mt = 0; mt < NT; ++mt) {
+      f32x16 acc[2] = {f32x16{}, f32x16{}};
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const bf16x8 A = wl[(mt * KB + kb) * WAVE + l];
+        acc[0] = mfma(A, h[kb][0], acc[0]);
+        acc[1] = mfma(A, h[kb][1], acc[1]);
+      }
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) t_epilogue(acc[ct], bl, mt, hi, hn[2 * mt][ct], hn[2 * mt + 1][ct]);
+    }
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      h[kb][0] = hn[kb][0];
+      h[kb][1] = hn[kb][1];
+    }
+  }
+  // output layer: one 32-feature tile, weights from L2
+  const bf16x8 *Wo = reinterpret_cast<const bf16x8 *>(a.Wsw[a.layers]);
+  f32x16 o[2] = {f32x16{}, f32x16{}};
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    const bf16x8 A = Wo[kb * WAVE + l];
+    o[0] = mfma(A, h[kb][0], o[0]);
+    o[1] = mfma(A, h[kb][1], o[1]);
+  }
+  const float *bo = a.bias[a.layers];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) {
+    const long long c = cell0 + 32 * ct + r32;
+    if (c >= a.n) continue;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int f = 8 * (r / 4) + 4 * hi + r % 4;
+      if (f < a.n_out) a.Y[c * a.n_out + f] = o[ct][r] + bo[f];
+    }
+  }
+}
+
 unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
@@ -416,7 +552,7 @@ struct nrg_mlp {
   std::vector<nrg::MlpLayer> L;            // layers + 1 entries
   long long n_pad = 0, n_packed = 0, n_sw = 0;
   float *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr;
-  nrg::bf16 *Pbf = nullptr, *Wsw = nullptr;
+  nrg::bf16 *Pbf = nullptr, *Wsw = nullptr, *WswT = nullptr;   // WswT: transposed-kernel layout
   int *gidx = nullptr;                      // packed index -> padded index
   long long adam_t = 0;
   bool sw_dirty = true;
@@ -471,10 +607,13 @@ int ensure_batch(nrg_mlp_t *h, long long n)
 int refresh_swizzle(nrg_mlp_t *h)
 {
   if (!h->sw_dirty) return NRG_CHEM_OK;
-  for (const MlpLayer &ly : h->L) {
+  for (size_t i = 0; i < h->L.size(); ++i) {
+    const MlpLayer &ly = h->L[i];
     const long long n = (long long)ly.K * ly.N;
     hipLaunchKernelGGL(swizzle_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream,
                        h->Pbf + ly.w_off, ly.K, ly.N, h->Wsw + ly.sw_off);
+    hipLaunchKernelGGL(swizzle_t_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream,
+                       h->Pbf + ly.w_off, ly.K, ly.N, i > 0 ? 1 : 0, h->WswT + ly.sw_off);
   }
   HIP_TRY(hipGetLastError());
   // inference may run on another stream (the chemistry handle's)
@@ -572,8 +711,9 @@ int mlp_infer_bf16(nrg_mlp_t *h, long long n, const bf16 *X, float *Y, hipStream
   if (rc) return rc;
   InferArgs a{};
   a.X = X; a.n = n; a.n_out = h->n_out; a.hidden = h->hidden; a.layers = h->layers; a.Y = Y;
+  const bool transposed = h->hidden == 256;
   for (int i = 0; i <= h->layers; ++i) {
-    a.Wsw[i] = h->Wsw + h->L[i].sw_off;
+    a.Wsw[i] = (transposed ? h->WswT : h->Wsw) + h->L[i].sw_off;
     a.bias[i] = h->P + h->L[i].b_off;
   }
   const dim3 grid(blocks_for(n, 128)), block(256);
@@ -581,7 +721,7 @@ int mlp_infer_bf16(nrg_mlp_t *h, long long n, const bf16 *X, float *Y, hipStream
     case 64: hipLaunchKernelGGL(mlp_infer_kernel<2>, grid, block, 0, stream, a); break;
     case 128: hipLaunchKernelGGL(mlp_infer_kernel<4>, grid, block, 0, stream, a); break;
     case 256:
-      hipLaunchKernelGGL(mlp_infer2_kernel<8>, dim3(blocks_for(n, 4 * ROWS2)), block, 0, stream, a);
+      hipLaunchKernelGGL(mlp_infer_t_kernel<8>, dim3(blocks_for(n, 4 * TCELLS)), block, 0, stream, a);
       break;
     default: return set_error(NRG_CHEM_E_ARG, "hidden width %d has no inference kernel", h->hidden);
   }
@@ -641,6 +781,7 @@ int nrg_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_o
   const size_t fb = h->n_pad * sizeof(float);
   if (hipMalloc(&h->P, fb) || hipMalloc(&h->G, fb) || hipMalloc(&h->M1, fb) || hipMalloc(&h->M2, fb) ||
       hipMalloc(&h->Pbf, h->n_pad * sizeof(bf16)) || hipMalloc(&h->Wsw, h->n_sw * sizeof(bf16)) ||
+      hipMalloc(&h->WswT, h->n_sw * sizeof(bf16)) ||
       hipMalloc(&h->gidx, h->n_packed * sizeof(int)) || hipMalloc(&h->loss_tmp, sizeof(float)))
     return fail(set_error(NRG_CHEM_E_HIP, "hipMalloc parameters"));
   std::vector<int> gi;
@@ -665,7 +806,7 @@ void nrg_mlp_destroy(nrg_mlp_t *h)
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (void *p : {(void *)h->P, (void *)h->G, (void *)h->M1, (void *)h->M2, (void *)h->Pbf,
-                  (void *)h->Wsw, (void *)h->gidx, (void *)h->loss_tmp, (void *)h->X, (void *)h->dY,
+                  (void *)h->Wsw, (void *)h->WswT, (void *)h->gidx, (void *)h->loss_tmp, (void *)h->X, (void *)h->dY,
                   (void *)h->Yb, (void *)h->dZ[0], (void *)h->dZ[1]})
     if (p) (void)hipFree(p);
   for (auto p : h->H) if (p) (void)hipFree(p);

@@ -654,37 +654,58 @@ __global__ void __launch_bounds__(256) featurize_kernel(SurArgs a)
   for (int i = 0; i < MLP_IN_PAD; ++i) x[i] = (bf16)f[i];
 }
 
-// Solve the n x n system M z = b in place (n <= 4, partial pivoting).
+// Solve the n x n system M z = b in place (n <= 4, partial pivoting).  Loops
+// run to the compile-time bound with n as a predicate, so M and b stay in
+// registers (row swaps are selects).
 NRG_DI bool small_solve(double (&M)[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS],
                         double (&b)[NRG_SUR_MAX_ELEMENTS], int n)
 {
-  for (int k = 0; k < n; ++k) {
+  constexpr int NE = NRG_SUR_MAX_ELEMENTS;
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    if (k >= n) break;
     int piv = k;
-    for (int i = k + 1; i < n; ++i)
-      if (fabs(M[i][k]) > fabs(M[piv][k])) piv = i;
-    if (M[piv][k] == 0.0) return false;
-    if (piv != k) {
-      for (int j = 0; j < n; ++j) {
+    double best = fabs(M[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < NE; ++i)
+      if (i < n && fabs(M[i][k]) > best) {
+        piv = i;
+        best = fabs(M[i][k]);
+      }
+    if (best == 0.0) ok = false;
+#pragma unroll
+    for (int i = k + 1; i < NE; ++i) {
+      const bool sw = (piv == i);
+#pragma unroll
+      for (int j = 0; j < NE; ++j) {
         const double t = M[k][j];
-        M[k][j] = M[piv][j];
-        M[piv][j] = t;
+        M[k][j] = sw ? M[i][j] : M[k][j];
+        M[i][j] = sw ? t : M[i][j];
       }
       const double t = b[k];
-      b[k] = b[piv];
-      b[piv] = t;
+      b[k] = sw ? b[i] : b[k];
+      b[i] = sw ? t : b[i];
     }
-    for (int i = k + 1; i < n; ++i) {
+#pragma unroll
+    for (int i = k + 1; i < NE; ++i) {
+      if (i >= n) break;
       const double f = M[i][k] / M[k][k];
-      for (int j = k; j < n; ++j) M[i][j] -= f * M[k][j];
+#pragma unroll
+      for (int j = k; j < NE; ++j) M[i][j] -= f * M[k][j];
       b[i] -= f * b[k];
     }
   }
-  for (int k = n - 1; k >= 0; --k) {
+#pragma unroll
+  for (int k = NE - 1; k >= 0; --k) {
+    if (k >= n) continue;
     double s = b[k];
-    for (int j = k + 1; j < n; ++j) s -= M[k][j] * b[j];
+#pragma unroll
+    for (int j = k + 1; j < NE; ++j)
+      if (j < n) s -= M[k][j] * b[j];
     b[k] = s / M[k][k];
   }
-  return true;
+  return ok;
 }
 
 __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
@@ -736,7 +757,9 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
     double res[NRG_SUR_MAX_ELEMENTS] = {0.0, 0.0, 0.0, 0.0};
     double d[S];
     if (reason == 0) {
-      for (int e = 0; e < a.ne; ++e) {
+#pragma unroll
+      for (int e = 0; e < NRG_SUR_MAX_ELEMENTS; ++e) {
+        if (e >= a.ne) break;
         double r = 0.0, sc = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
@@ -752,8 +775,11 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
       double M[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS] = {};
 #pragma unroll
       for (int s = 0; s < S; ++s) d[s] = fabs(dy[s]) + 1e-12;
-      for (int e = 0; e < a.ne; ++e)
-        for (int f = 0; f < a.ne; ++f) {
+#pragma unroll
+      for (int e = 0; e < NRG_SUR_MAX_ELEMENTS; ++e)
+#pragma unroll
+        for (int f = 0; f < NRG_SUR_MAX_ELEMENTS; ++f) {
+          if (e >= a.ne || f >= a.ne) continue;
           double m = 0.0;
 #pragma unroll
           for (int s = 0; s < S; ++s) m += a.E[e][s] * a.E[f][s] * d[s] / (mc.W[s] * mc.W[s]);
@@ -763,7 +789,9 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 #pragma unroll
         for (int s = 0; s < S; ++s) {
           double c = 0.0;
-          for (int e = 0; e < a.ne; ++e) c += a.E[e][s] * res[e];
+#pragma unroll
+          for (int e = 0; e < NRG_SUR_MAX_ELEMENTS; ++e)
+            if (e < a.ne) c += a.E[e][s] * res[e];
           dy[s] -= d[s] * c / mc.W[s];
         }
       }

@@ -1,8 +1,10 @@
 """BF16 MFMA perceptron kernels (include/nrg_mlp.h) against the FP64
 reference (oracle/mlp_fp64.py), through the C-ABI.
 
-Bars: the fused inference kernel and the training-path forward agree to
-within FP32 rounding (same MFMA accumulation order); both agree with the FP64
+Bars: the fused inference kernel (hidden width 256: activations in registers,
+each hidden layer's K rows in the order its predecessor's accumulators hold
+them, so its FP32 sums run in another order than the training GEMMs') and the
+training-path forward agree to 2e-3 norm-wise; both agree with the FP64
 reference that rounds to bf16 where the kernels do to 5e-3 (norm-wise); the
 gradient of every layer agrees with that reference to 2e-2 norm-wise (bf16
 rounding of activations and back-propagated gradients, FP32 accumulation,
@@ -55,7 +57,7 @@ def test_forward_matches_reference(n):
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     _report("forward", n=n, inf_vs_train=rel(y_inf, y_tr), inf_vs_ref=rel(y_inf, y_ref),
             inf_vs_fp64=rel(y_inf, y_fp64))
-    assert np.abs(y_inf - y_tr).max() <= 1e-5 * np.abs(y_tr).max()
+    assert rel(y_inf, y_tr) <= 2e-3
     assert rel(y_inf, y_ref) <= 5e-3
     assert rel(y_inf, y_fp64) <= 5e-2
     net.close()
