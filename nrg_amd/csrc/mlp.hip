@@ -404,7 +404,12 @@ __global__ void __launch_bounds__(256) mlp_infer2_kernel(InferArgs a)
 // weights are stored with their K rows in that order (swizzle_t_kernel).  So
 // hidden activations never leave the registers; the hidden layers' weights
 // (H x H bf16, 128 KB at H = 256) are staged in LDS once per layer and block.
-constexpr int TCELLS = 64;          // cells per wave: two 32-cell tiles share each A fragment
+#ifndef T_NC
+#define T_NC 1                      // 32-cell tiles per wave (they share each A fragment)
+#endif
+#ifndef T_WPB
+#define T_WPB 8                     // waves per block: two per SIMD, so one wave's GELUs
+#endif                              // overlap the other's MFMAs
 
 // physical feature (output column of the previous layer) of logical K row kl
 __host__ __device__ inline int t_phys(int kl)
@@ -430,14 +435,22 @@ __global__ void swizzle_t_kernel(const bf16 *W, int K, int N, int permuted, bf16
   WswT[i] = W[(long long)k * N + 32 * mt + lane % 32];
 }
 
-// bias + GELU of one 32-feature tile, packed as the next layer's two B fragments
-__device__ __forceinline__ void t_epilogue(const f32x16 &acc, const float *bias, int mt, int hi,
-                                           bf16x8 &lo, bf16x8 &up)
+// Accumulator start value: the bias of the lane's 16 output features of tile mt
+// (so the epilogue needs no separate add).
+__device__ __forceinline__ f32x16 t_bias(const float *bias, int mt, int hi)
+{
+  f32x16 b;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) b[r] = bias[32 * mt + 8 * (r / 4) + 4 * hi + r % 4];
+  return b;
+}
+
+// GELU of one 32-feature tile, packed as the next layer's two B fragments
+__device__ __forceinline__ void t_epilogue(const f32x16 &acc, bf16x8 &lo, bf16x8 &up)
 {
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const float z = acc[r] + bias[32 * mt + 8 * (r / 4) + 4 * hi + r % 4];
-    const bf16 g = (bf16)gelu(z);
+    const bf16 g = (bf16)gelu(acc[r]);
     if (r < 8)
       lo[r] = g;
     else
@@ -445,40 +458,44 @@ __device__ __forceinline__ void t_epilogue(const f32x16 &acc, const float *bias,
   }
 }
 
-template <int NT>
-__global__ void __launch_bounds__(256) mlp_infer_t_kernel(InferArgs a)
+// NC 32-cell tiles per wave, WPB waves per block.
+template <int NT, int NC, int WPB>
+__global__ void __launch_bounds__(WPB * 64) mlp_infer_t_kernel(InferArgs a)
 {
   constexpr int KB = 2 * NT;                      // k-blocks of a hidden layer
+  constexpr int NTHR = WPB * WAVE;
   __shared__ bf16x8 wl[NT * KB * WAVE];           // one hidden layer's A fragments
   __shared__ float bl[32 * NT];                   // its bias
   const int w = threadIdx.x / WAVE, l = threadIdx.x & (WAVE - 1);
   const int r32 = l & 31, hi = l >> 5;
-  const long long cell0 = (long long)blockIdx.x * (4 * TCELLS) + w * TCELLS;
+  const long long cell0 = ((long long)blockIdx.x * WPB + w) * (32 * NC);
 
-  bf16x8 h[KB][2];
+  bf16x8 h[KB][NC];
   // layer 0 (K = 32): input B fragments straight from X, weights from L2
   {
-    bf16x8 x[2][2];
+    bf16x8 x[2][NC];
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
+    for (int ct = 0; ct < NC; ++ct) {
       const long long c = cell0 + 32 * ct + r32 < a.n ? cell0 + 32 * ct + r32 : a.n - 1;
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb)
         x[kb][ct] = *reinterpret_cast<const bf16x8 *>(a.X + c * MLP_IN_PAD + 16 * kb + 8 * hi);
     }
     const bf16x8 *W0 = reinterpret_cast<const bf16x8 *>(a.Wsw[0]);
-    const float *b0 = a.bias[0];
 #pragma unroll
     for (int mt = 0; mt < NT; ++mt) {
-      f32x16 acc[2] = {f32x16{}, f32x16{}};
+      const f32x16 b = t_bias(a.bias[0], mt, hi);
+      f32x16 acc[NC];
+#pragma unroll
+      for (int ct = 0; ct < NC; ++ct) acc[ct] = b;
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb) {
         const bf16x8 A = W0[(mt * 2 + kb) * WAVE + l];
-        acc[0] = mfma(A, x[kb][0], acc[0]);
-        acc[1] = mfma(A, x[kb][1], acc[1]);
+#pragma unroll
+        for (int ct = 0; ct < NC; ++ct) acc[ct] = mfma(A, x[kb][ct], acc[ct]);
       }
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) t_epilogue(acc[ct], b0, mt, hi, h[2 * mt][ct], h[2 * mt + 1][ct]);
+      for (int ct = 0; ct < NC; ++ct) t_epilogue(acc[ct], h[2 * mt][ct], h[2 * mt + 1][ct]);
     }
   }
   // hidden layers
@@ -486,46 +503,53 @@ __global__ void __launch_bounds__(256) mlp_infer_t_kernel(InferArgs a)
     __syncthreads();                               // previous layer's weights no longer read
     const bf16x8 *W = reinterpret_cast<const bf16x8 *>(a.Wsw[layer]);
 #pragma unroll 4
-    for (int i = threadIdx.x; i < NT * KB * WAVE; i += 256) wl[i] = W[i];
-    if (threadIdx.x < 32 * NT) bl[threadIdx.x] = a.bias[layer][threadIdx.x];
+    for (int i = threadIdx.x; i < NT * KB * WAVE; i += NTHR) wl[i] = W[i];
+    for (int i = threadIdx.x; i < 32 * NT; i += NTHR) bl[i] = a.bias[layer][i];
     __syncthreads();
-    bf16x8 hn[KB][2];
+    bf16x8 hn[KB][NC];
 #pragma unroll
     for (int mt = 0; mt < NT; ++mt) {
-      f32x16 acc[2] = {f32x16{}, f32x16{}};
+      const f32x16 b = t_bias(bl, mt, hi);
+      f32x16 acc[NC];
+#pragma unroll
+      for (int ct = 0; ct < NC; ++ct) acc[ct] = b;
 #pragma unroll
       for (int kb = 0; kb < KB; ++kb) {
         const bf16x8 A = wl[(mt * KB + kb) * WAVE + l];
-        acc[0] = mfma(A, h[kb][0], acc[0]);
-        acc[1] = mfma(A, h[kb][1], acc[1]);
+#pragma unroll
+        for (int ct = 0; ct < NC; ++ct) acc[ct] = mfma(A, h[kb][ct], acc[ct]);
       }
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) t_epilogue(acc[ct], bl, mt, hi, hn[2 * mt][ct], hn[2 * mt + 1][ct]);
+      for (int ct = 0; ct < NC; ++ct) t_epilogue(acc[ct], hn[2 * mt][ct], hn[2 * mt + 1][ct]);
     }
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      h[kb][0] = hn[kb][0];
-      h[kb][1] = hn[kb][1];
-    }
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int ct = 0; ct < NC; ++ct) h[kb][ct] = hn[kb][ct];
   }
   // output layer: one 32-feature tile, weights from L2
   const bf16x8 *Wo = reinterpret_cast<const bf16x8 *>(a.Wsw[a.layers]);
-  f32x16 o[2] = {f32x16{}, f32x16{}};
+  const float *bo = a.bias[a.layers];
+  f32x16 o[NC];
+  {
+    const f32x16 b = t_bias(bo, 0, hi);
+#pragma unroll
+    for (int ct = 0; ct < NC; ++ct) o[ct] = b;
+  }
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
     const bf16x8 A = Wo[kb * WAVE + l];
-    o[0] = mfma(A, h[kb][0], o[0]);
-    o[1] = mfma(A, h[kb][1], o[1]);
-  }
-  const float *bo = a.bias[a.layers];
 #pragma unroll
-  for (int ct = 0; ct < 2; ++ct) {
+    for (int ct = 0; ct < NC; ++ct) o[ct] = mfma(A, h[kb][ct], o[ct]);
+  }
+#pragma unroll
+  for (int ct = 0; ct < NC; ++ct) {
     const long long c = cell0 + 32 * ct + r32;
     if (c >= a.n) continue;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int f = 8 * (r / 4) + 4 * hi + r % 4;
-      if (f < a.n_out) a.Y[c * a.n_out + f] = o[ct][r] + bo[f];
+      if (f < a.n_out) a.Y[c * a.n_out + f] = o[ct][r];
     }
   }
 }
@@ -721,7 +745,8 @@ int mlp_infer_bf16(nrg_mlp_t *h, long long n, const bf16 *X, float *Y, hipStream
     case 64: hipLaunchKernelGGL(mlp_infer_kernel<2>, grid, block, 0, stream, a); break;
     case 128: hipLaunchKernelGGL(mlp_infer_kernel<4>, grid, block, 0, stream, a); break;
     case 256:
-      hipLaunchKernelGGL(mlp_infer_t_kernel<8>, dim3(blocks_for(n, 4 * TCELLS)), block, 0, stream, a);
+      hipLaunchKernelGGL((mlp_infer_t_kernel<8, T_NC, T_WPB>), dim3(blocks_for(n, 32 * T_NC * T_WPB)),
+                         dim3(64 * T_WPB), 0, stream, a);
       break;
     default: return set_error(NRG_CHEM_E_ARG, "hidden width %d has no inference kernel", h->hidden);
   }

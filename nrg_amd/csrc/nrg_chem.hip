@@ -708,9 +708,13 @@ NRG_DI bool small_solve(double (&M)[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS],
   return ok;
 }
 
+// The per-reason counts are summed per block in LDS and added to the global
+// counters once per block.
 __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 {
   __shared__ MechCoef mc;
+  __shared__ unsigned long long blk[SUR_SLOTS];
+  if (threadIdx.x < SUR_SLOTS) blk[threadIdx.x] = 0ull;
   stage_mech(mc, a.mech);
   const int lane = threadIdx.x & (WAVE - 1);
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -857,21 +861,23 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
     base = __shfl(base, leader, WAVE);
     if (rej) a.list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)q;
   }
-  const unsigned long long n_acc = __popcll(__ballot(live && reason <= 0 ? 1 : 0));
-  const unsigned long long n_frz = __popcll(__ballot(live && reason < 0 ? 1 : 0));
-  const unsigned long long n_gat = __popcll(__ballot(gated ? 1 : 0));
-  unsigned long long n_r[SUR_NREASONS];
+  const unsigned long long c_acc = __popcll(__ballot(live && reason <= 0 ? 1 : 0));
+  const unsigned long long c_frz = __popcll(__ballot(live && reason < 0 ? 1 : 0));
+  const unsigned long long c_gat = __popcll(__ballot(gated ? 1 : 0));
+  unsigned long long c_r[SUR_NREASONS];
 #pragma unroll
   for (int i = 0; i < SUR_NREASONS; ++i)
-    n_r[i] = __popcll(__ballot(rej && reason == SUR_R_DOMAIN + i ? 1 : 0));
+    c_r[i] = __popcll(__ballot(rej && reason == SUR_R_DOMAIN + i ? 1 : 0));
   if (lane == 0) {
-    if (n_acc) atomicAdd(&a.acc[SUR_ACCEPT], n_acc);
-    if (n_gat) atomicAdd(&a.acc[SUR_GATED], n_gat);
-    if (n_frz) atomicAdd(&a.acc[SUR_FROZEN], n_frz);
+    atomicAdd(&blk[SUR_ACCEPT], c_acc);
+    atomicAdd(&blk[SUR_GATED], c_gat);
+    atomicAdd(&blk[SUR_FROZEN], c_frz);
 #pragma unroll
-    for (int i = 0; i < SUR_NREASONS; ++i)
-      if (n_r[i]) atomicAdd(&a.acc[SUR_R_DOMAIN + i], n_r[i]);
+    for (int i = 0; i < SUR_NREASONS; ++i) atomicAdd(&blk[SUR_R_DOMAIN + i], c_r[i]);
   }
+  __syncthreads();
+  if (threadIdx.x < SUR_SLOTS && threadIdx.x != SUR_FALLBACK && blk[threadIdx.x])
+    atomicAdd(&a.acc[threadIdx.x], blk[threadIdx.x]);
 }
 
 }  // namespace

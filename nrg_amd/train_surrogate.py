@@ -33,7 +33,18 @@ def main(argv=None):
                     help="only calibrate the gate of this trained file and write it to --out")
     ap.add_argument("--target-dy", type=float, default=1e-4,
                     help="max one-step error of accepted major species against FP64")
+    ap.add_argument("--set-gate", default=None, metavar="FILE",
+                    help="only rewrite the gate thresholds of this trained file (--gate K=V ...) "
+                         "into --out")
+    ap.add_argument("--gate", action="append", default=[], metavar="K=V",
+                    help="gate threshold for --set-gate, e.g. dT_max=10")
     a = ap.parse_args(argv)
+    if a.set_gate:
+        from nrg_amd import surrogate as sg
+        kw = dict((k, float(v)) for k, v in (g.split("=", 1) for g in a.gate))
+        os.makedirs(a.out, exist_ok=True)
+        sg.set_gate(a.set_gate, os.path.join(a.out, "surrogate_keromnes.npz"), **kw)
+        return 0
 
     import torch
 

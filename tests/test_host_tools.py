@@ -49,3 +49,26 @@ def test_bench_reads_profiled_traffic(tmp_path, monkeypatch):
     assert t["bytes_per_call"] == 2 * 1600
     assert set(t["per_dispatch"]) == {"chem_bdf_kernel FETCH_SIZE", "chem_bdf_kernel WRITE_SIZE",
                                       "rates_kernel WRITE_SIZE"}
+
+
+def test_set_gate_rewrites_only_the_thresholds(tmp_path):
+    """train_surrogate --set-gate: the named gate thresholds change, every other
+    array of the trained file (parameters, normalizer, metadata) is kept."""
+    import numpy as np
+    from nrg_amd import train_surrogate
+    src = tmp_path / "in.npz"
+    np.savez(src, params=np.arange(7, dtype=np.float32), in_shift=np.ones(12, np.float32),
+             gate_dT_max=np.float64(0.25), gate_energy_tol=np.float64(0.1),
+             meta=np.array(repr({"seed": 3})))
+    out = tmp_path / "out"
+    assert train_surrogate.main(["--out", str(out), "--set-gate", str(src),
+                                 "--gate", "dT_max=10"]) == 0
+    with np.load(src) as a, np.load(out / "surrogate_keromnes.npz") as b:
+        assert sorted(a.files) == sorted(b.files)
+        assert float(b["gate_dT_max"]) == 10.0
+        for k in a.files:
+            if k != "gate_dT_max":
+                assert np.array_equal(a[k], b[k]), k
+    import pytest
+    with pytest.raises(KeyError):
+        train_surrogate.main(["--out", str(out), "--set-gate", str(src), "--gate", "nope=1"])
