@@ -14,6 +14,7 @@ kernel measured with HIP events, CPU baseline from the oracle on a sample).
 """
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -196,7 +197,7 @@ def main():
     chunk_cells = (tuning["best"]["chunk_cells"] if tuning else
                    (1 << (args.chunk_log2 or 23)))
     nchunks = -(-ncells // chunk_cells)
-    tr = profiled_traffic(nchunks)
+    tr = profiled_traffic(ncells)
     roofline = {
         "kernel": "rates_kernel + chem_bdf_kernel (one chemistry step)",
         "bound": "mfma",
@@ -284,13 +285,20 @@ PEAK_BF16_TFLOPS = 2500.0
 HBM_PROFILE = os.path.join(ROOT, "profiles", "chem_step_256cube_hbm_bytes.txt")
 
 
-def profiled_traffic(nchunks, kernels=("chem_bdf_kernel", "rates_kernel")):
+def profiled_traffic(cells_per_call, kernels=("chem_bdf_kernel", "rates_kernel")):
     """HBM bytes of one chemistry call from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (per-dispatch bytes, copy-calibrated) x dispatches per call."""
+    WRITE_SIZE summary: per-dispatch bytes (copy-calibrated) of the profiled
+    run, whose dispatches covered 2^k cells (its command's --chunk-log2 k),
+    scaled to the cells of this call."""
     if not os.path.exists(HBM_PROFILE):
         return None
     per = {}
+    k_log2 = 23
     for line in open(HBM_PROFILE):
+        if line.startswith("#"):
+            m = re.search(r"--chunk-log2 (\d+)", line)
+            if m:
+                k_log2 = int(m.group(1))
         if line.startswith("TRAFFIC"):
             _, k, c, b = line.rstrip("\n").split("\t")
             for want in kernels:
@@ -298,10 +306,12 @@ def profiled_traffic(nchunks, kernels=("chem_bdf_kernel", "rates_kernel")):
                     per[(want, c)] = float(b)
     if not per:
         return None
-    total = sum(per.values()) * nchunks
-    return {"bytes_per_call": total,
+    prof_cells = 1 << k_log2
+    scale = cells_per_call / prof_cells
+    return {"bytes_per_call": sum(per.values()) * scale,
             "per_dispatch": {"%s %s" % kc: v for kc, v in sorted(per.items())},
-            "dispatches_per_call": nchunks, "source": os.path.relpath(HBM_PROFILE, ROOT)}
+            "profiled_dispatch_cells": prof_cells, "cells_per_call": cells_per_call,
+            "source": os.path.relpath(HBM_PROFILE, ROOT)}
 
 
 def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, fp64_value, S):

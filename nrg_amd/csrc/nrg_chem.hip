@@ -514,7 +514,9 @@ struct BinArgs {
   const int32_t *hist;              // NSTEP of the previous call, or NULL (use T)
   unsigned long long *count;        // [NBIN]
   unsigned long long *cursor;       // [NBIN] first queue position of each bin
-  int32_t *list;                    // [ninner]
+  const int32_t *in_list;           // cells to order (NULL: all inner cells 0..npos-1)
+  long long npos;                   // number of cells to order
+  int32_t *list;                    // [npos] the ordered cells
 };
 
 // 0 = gated (no integration); otherwise grows with the expected cost.
@@ -538,10 +540,10 @@ NRG_DI void bin_histogram(const BinArgs &a, unsigned int *h)
 {
   if (threadIdx.x < NBIN) h[threadIdx.x] = 0u;
   __syncthreads();
-  const long long q0 = (long long)blockIdx.x * BIN_RUN + threadIdx.x;
+  const long long p0 = (long long)blockIdx.x * BIN_RUN + threadIdx.x;
   for (int k = 0; k < BIN_RUN / BIN_THREADS; ++k) {
-    const long long q = q0 + (long long)k * BIN_THREADS;
-    if (q < a.g.ninner) atomicAdd(&h[cost_bin(a, q)], 1u);
+    const long long p = p0 + (long long)k * BIN_THREADS;
+    if (p < a.npos) atomicAdd(&h[cost_bin(a, a.in_list ? (long long)a.in_list[p] : p)], 1u);
   }
   __syncthreads();
 }
@@ -577,10 +579,11 @@ __global__ void __launch_bounds__(BIN_THREADS) bin_scatter_kernel(BinArgs a)
     h[threadIdx.x] = 0u;
   }
   __syncthreads();
-  const long long q0 = (long long)blockIdx.x * BIN_RUN + threadIdx.x;
+  const long long p0 = (long long)blockIdx.x * BIN_RUN + threadIdx.x;
   for (int k = 0; k < BIN_RUN / BIN_THREADS; ++k) {
-    const long long q = q0 + (long long)k * BIN_THREADS;
-    if (q < a.g.ninner) {
+    const long long p = p0 + (long long)k * BIN_THREADS;
+    if (p < a.npos) {
+      const long long q = a.in_list ? (long long)a.in_list[p] : p;
       const int b = cost_bin(a, q);
       const unsigned int rank = atomicAdd(&h[b], 1u);
       a.list[base[b] + rank] = (int32_t)q;
@@ -1248,9 +1251,12 @@ static int prepare_order(nrg_chem_t *h, long long n)
   return NRG_CHEM_OK;
 }
 
-static void enqueue_order(nrg_chem_t *h, StepArgs &a, bool use_hist)
+// Order the cells in_list[0..n) (all inner cells when in_list is NULL) into
+// h->order_list, most expensive bin first.
+static void enqueue_order(nrg_chem_t *h, StepArgs &a, bool use_hist, const int32_t *in_list = nullptr,
+                          long long n = -1)
 {
-  const long long n = a.g.ninner;
+  if (n < 0) n = a.g.ninner;
   BinArgs b{};
   b.g = a.g;
   b.T = a.T;
@@ -1260,6 +1266,8 @@ static void enqueue_order(nrg_chem_t *h, StepArgs &a, bool use_hist)
   b.count = h->bins;
   b.cursor = h->bins + NBIN;
   b.list = h->order_list;
+  b.in_list = in_list;
+  b.npos = n;
   const unsigned nb = (unsigned)((n + BIN_RUN - 1) / BIN_RUN);
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(2 * NBIN), 0, h->stream, h->bins, 2 * NBIN);
   hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(BIN_THREADS), 0, h->stream, b);
@@ -1845,6 +1853,15 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     if (nfb > n) return set_error(NRG_CHEM_E_HIP, "fallback count %lld exceeds %lld cells", nfb, n);
     if (nfb > 0) {
       a.list = h->sur_list;
+      // the fallback cells are the stiffest of the field: with a stiffness
+      // history for this grid, hand them out most expensive first (and
+      // record their step counts for the next call)
+      if (h->order == NRG_ORDER_STIFFNESS && h->hist_valid && same_grid(h->hist_grid, a.g) &&
+          h->order_cells >= n) {
+        enqueue_order(h, a, true, h->sur_list, nfb);
+        a.list = h->order_list;
+        a.hist = h->hist;
+      }
       if ((st = enqueue_integrator(h, a, nfb))) return st;
     }
     HIP_TRY(hipEventRecord(h->ev1, h->stream));

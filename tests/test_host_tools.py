@@ -41,12 +41,14 @@ def test_bench_reads_profiled_traffic(tmp_path, monkeypatch):
     sys.path.insert(0, ROOT)
     bench = importlib.import_module("bench")
     f = tmp_path / "hbm.txt"
-    f.write_text("# summary\nTRAFFIC\tchem_bdf_kernel\tFETCH_SIZE\t1000\n"
+    f.write_text("# command: rocprofv3 ... -- python3 bench.py --chunk-log2 1\n"
+                 "TRAFFIC\tchem_bdf_kernel\tFETCH_SIZE\t1000\n"
                  "TRAFFIC\tchem_bdf_kernel\tWRITE_SIZE\t500\nTRAFFIC\trates_kernel\tWRITE_SIZE\t100\n"
                  "TRAFFIC\tclosure_kernel\tWRITE_SIZE\t7\n")
     monkeypatch.setattr(bench, "HBM_PROFILE", str(f))
-    t = bench.profiled_traffic(2)
+    t = bench.profiled_traffic(4)           # 4 cells, profiled dispatches of 2 cells
     assert t["bytes_per_call"] == 2 * 1600
+    assert t["profiled_dispatch_cells"] == 2
     assert set(t["per_dispatch"]) == {"chem_bdf_kernel FETCH_SIZE", "chem_bdf_kernel WRITE_SIZE",
                                       "rates_kernel WRITE_SIZE"}
 
