@@ -177,3 +177,27 @@ def test_element_and_energy_gates_reject(engine, mech, zero_net):
     s = o["stats"]
     _report("gates", reject_energy=s.reject_energy, live=live)
     assert s.reject_energy == live
+
+
+def test_fallback_in_stiffness_order_is_bitwise_identical(engine, mech, zero_net):
+    """With NRG_ORDER_STIFFNESS and a step-count history for the grid, the
+    fallback cells are handed out most expensive first; outputs and work
+    counts equal those of the unordered fallback bit for bit, over two calls
+    (the second uses the counts the first recorded)."""
+    from nrg_amd.solver import ChemistryStep
+    st, bc = _state(mech, seed=13)
+    eng = ChemistryStep(mech, capi.default_opts(cell_order=capi.NRG_ORDER_STIFFNESS))
+    try:
+        _fp64(eng, st, bc, 1e-7)                       # history for this grid
+        gate = _gate(mech, t_range=(1500.0, 1e4))
+        for call in range(2):
+            a = _run(engine, zero_net, _norm(np.zeros(S + 1)), gate, st, bc, 1e-7)
+            b = _run(eng, zero_net, _norm(np.zeros(S + 1)), gate, st, bc, 1e-7)
+            _report("fallback_order", call=call, fallback=b["stats"].cells_fallback,
+                    nfe=b["stats"].fallback.nfe_sum)
+            assert b["stats"].cells_fallback > 0
+            for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+                assert np.array_equal(a[k], b[k]), (call, k)
+            assert a["stats"].fallback.nfe_sum == b["stats"].fallback.nfe_sum
+    finally:
+        eng.close()
