@@ -274,7 +274,8 @@ NRG_DI double e_molar(const MechCoef &mc, double T, int s)
 // kout[r*stride] = k_f(r), kout[(R+r)*stride] = k_r(r), and [M] of the
 // reactions whose RHS term uses it, kout[(2R+slot)*stride] (:235-248).
 // ---------------------------------------------------------------------------
-template <class M>
+// Reactions [R0, R1) only (their k_f, k_r and [M] slots).
+template <class M, int R0 = 0, int R1 = R>
 NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], double *kout,
                            long long stride)
 {
@@ -286,7 +287,7 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
     hv[c] = h_molar(mc, T, c);
   }
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int r = R0; r < R1; ++r) {
     double m = 0.0;
     if (uses_m(r)) {
 #pragma unroll

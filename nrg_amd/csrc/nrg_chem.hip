@@ -173,16 +173,23 @@ NRG_DI bool start_cell(const StepArgs &a, const BdfCoef &mc, long long t, long l
   return true;
 }
 
-// Rate constants and [M] of every integrated cell of the chunk, one cell per
-// thread (chemical_kinetics_solver.f90:235-252, 360-493).  Kept apart from the
-// integrator so its ~80 transcendentals run coherently at two waves per SIMD
-// (the register budget that allows costs a few spilled values; one wave per
-// SIMD, the unconstrained allocation, measured 3 ms slower per 2^24 cells).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) rates_kernel(StepArgs a)
+// Rate constants and [M] of every integrated cell of the chunk
+// (chemical_kinetics_solver.f90:235-252, 360-493), kept apart from the
+// integrator so its ~80 transcendentals run coherently.  A block covers
+// RATES_CELLS cells RATES_GROUPS times: the waves of group g compute
+// reactions [g R / G, (g+1) R / G) (a wave-uniform split, so each wave's code
+// and registers cover a part of the mechanism: occupancy 4 instead of 1-2).
+#ifndef RATES_GROUPS
+#define RATES_GROUPS 2
+#endif
+constexpr int RATES_CELLS = 256 / RATES_GROUPS;
+
+__global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
 {
   __shared__ MechCoef mc;
   stage_mech(mc, a.mech);
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / RATES_CELLS);   // wave-uniform
+  const long long t = (long long)blockIdx.x * RATES_CELLS + (threadIdx.x % RATES_CELLS);
   if (t >= a.nchunk) return;
   const long long q = cell_of(a, t);
   const long long off = cell_offset(a.g, q);
@@ -193,7 +200,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) r
   double C[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) / mc.W[s];
-  rate_constants<InlineMath>(mc, T, C, a.rates_out + t, a.nchunk);
+  constexpr int G = RATES_GROUPS;
+  static_assert(G == 1 || G == 2 || G == 4, "RATES_GROUPS must be 1, 2 or 4");
+  double *out = a.rates_out + t;
+  if (G == 1 || grp == 0)
+    rate_constants<InlineMath, 0, R / G>(mc, T, C, out, a.nchunk);
+  else if (G == 2 || grp == 1)
+    rate_constants<InlineMath, R / G, 2 * R / G>(mc, T, C, out, a.nchunk);
+  else if (grp == 2)
+    rate_constants<InlineMath, 2 * R / G, 3 * R / G>(mc, T, C, out, a.nchunk);
+  else
+    rate_constants<InlineMath, 3 * R / G, R>(mc, T, C, out, a.nchunk);
 }
 
 // Output assembly (:318-340) once the cell's DDRIV3 has returned; returns the
@@ -271,11 +288,14 @@ struct Tally {
 
 // One chemistry step (solve_chemical_kinetics, :224-346), one cell per lane.
 // PERSISTENT = false: lane i of block b owns inner cell 64*b + i (the plain
-//   launch).  PERSISTENT = true: a resident grid; a lane that finishes takes
-//   the next unclaimed cell from a.queue (one atomic per wave per refill).
+//   launch).  PERSISTENT = true: a resident grid; once REFILL_MIN_IDLE lanes
+//   of a wave have finished (or all have), they take the next unclaimed cells
+//   from a.queue (one atomic per wave per refill).
 // Each iteration either evaluates f for the lanes that asked, or -- when at
 // least jac_pct % of the pending lanes wait for a Jacobian, or nothing else is
 // pending -- runs one batched FD Jacobian + LU for the waiting lanes.
+constexpr int REFILL_MIN_IDLE = 48;    // persistent form: idle lanes that trigger a refill
+
 #ifndef NRG_BDF_WAVES_PER_EU
 #define NRG_BDF_WAVES_PER_EU 1   // resident waves per SIMD the register budget is sized for
 #endif
@@ -311,7 +331,10 @@ chem_bdf_kernel(StepArgs a)
   }
   for (;;) {
     if (PERSISTENT) {
+      // refill in batches: new cells start together, so the wave's lanes stay
+      // near the same stage (refilling every pass measured 30 % slower)
       bool need = !have && !exhausted;
+      if (wave_any(have) && __popcll(__ballot(need ? 1 : 0)) < REFILL_MIN_IDLE) need = false;
       while (wave_any(need)) {
         const unsigned long long mask = __ballot(need ? 1 : 0);
         const int leader = __ffsll((long long)mask) - 1;
@@ -1310,8 +1333,8 @@ static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
     a.base = ci * chunk;
     a.nchunk = std::min<long long>(chunk, npos - a.base);
     a.queue = h->dev_acc + ACC_QUEUE0 + ci;
-    hipLaunchKernelGGL(rates_kernel, dim3((unsigned)((a.nchunk + 255) / 256)), dim3(256), 0,
-                       h->stream, a);
+    hipLaunchKernelGGL(rates_kernel, dim3((unsigned)((a.nchunk + RATES_CELLS - 1) / RATES_CELLS)),
+                       dim3(256), 0, h->stream, a);
     const long long need = (a.nchunk + WAVE - 1) / WAVE;
     const bool an = h->opts.jacobian == NRG_JAC_ANALYTIC;
     if (h->form == NRG_FORM_PERSISTENT) {
