@@ -290,6 +290,31 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt,
                           double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
                           nrg_chem_tuning *out);
 
+/*
+ * Table-approximated chemistry: the reference's ODE_solver = 'table_approximated'
+ * mode (computing_module/src/current_build/chemical_kinetics_solver.f90:168-169,
+ * 250-346, interpolate_chemical_kinetics :650-685).
+ *
+ * nrg_chem_set_table replaces read_chemical_kinetics_table (:608-648): `rows`
+ * temperatures T_t and, species-major, dC_t[s*rows + i] (the table's columns
+ * 2..S+1, i.e. Y_prod2 of the run that wrote it, write_chemical_kinetics_table
+ * :542-606).  The table is copied to the device.
+ *
+ * nrg_chem_table_step_dev has nrg_chem_step_dev's arguments and gate (p and
+ * rho are not read).  For an integrated cell: C_in = Y; C_out = the table
+ * row pair bracketing T, interpolated linearly (the last bracketing interval
+ * wins; T below the table gives 0, above it the last row; no bracketing
+ * interval leaves C_out = Y); Y_prod = C_out*W/dt, Y_prod2 = C_out - C_in,
+ * E_f_prod = -sum_s h_s(T_ref)*C_out/dt, status NRG_CELL_OK.  stats carries
+ * cells_inner, cells_integrated and kernel_ms.
+ */
+int nrg_chem_set_table(nrg_chem_t *h, int32_t rows, const double *T_t, const double *dC_t);
+int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt,
+                            const double *T, const double *p, const double *rho,
+                            const double *const *Y, const int8_t *bc,
+                            double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                            int32_t *cell_status, nrg_chem_stats *stats);
+
 /* HIP stream the handle launches on (a hipStream_t), for callers that time or order work. */
 void *nrg_chem_stream(nrg_chem_t *h);
 

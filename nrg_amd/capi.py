@@ -322,6 +322,10 @@ def load_library(path=None):
     lib.nrg_chem_stream.restype = vp
     lib.nrg_chem_autotune_dev.argtypes = step_args[:-2] + [C.POINTER(NrgChemTuning)]
     lib.nrg_chem_autotune_dev.restype = C.c_int
+    lib.nrg_chem_set_table.argtypes = [vp, C.c_int32, vp, vp]
+    lib.nrg_chem_set_table.restype = C.c_int
+    lib.nrg_chem_table_step_dev.argtypes = step_args
+    lib.nrg_chem_table_step_dev.restype = C.c_int
     # include/nrg_mlp.h
     lib.nrg_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                    C.POINTER(vp)]

@@ -520,6 +520,86 @@ __global__ void __launch_bounds__(256) energy_kernel(EnergyArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Table-approximated chemistry (chemical_kinetics_solver.f90:250-346 with
+// ODE_solver = 'table_approximated', interpolate_chemical_kinetics :650-685):
+// the "concentrations" are the mass fractions themselves, replaced by the
+// table's dC(T) interpolated linearly in T (the last bracketing interval
+// wins; below the table 0, above it the last row); then Y_prod = C_out*W/dt,
+// Y_prod2 = C_out - Y, E_f_prod = -sum_s h_s(T_ref)*C_out/dt.
+// ---------------------------------------------------------------------------
+struct TableArgs {
+  GridIdx g;
+  double dt, t_gate;
+  const MechCoef *mech;
+  const double *T;
+  const double *Y[S];
+  const int8_t *bc;
+  double *Yp[S], *Yp2[S];
+  double *Efp;
+  int32_t *status;
+  const double *table;      // T_t[rows], dC_t[S][rows]
+  int rows;
+  unsigned long long *acc;  // [0] integrated cells
+};
+
+__global__ void __launch_bounds__(256) table_kernel(TableArgs a)
+{
+  __shared__ double W[S], href[S];
+  if (threadIdx.x < S) {
+    W[threadIdx.x] = a.mech->W[threadIdx.x];
+    href[threadIdx.x] = a.mech->h_ref[threadIdx.x];
+  }
+  __syncthreads();
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool integ = false;
+  if (q < a.g.ninner) {
+    const long long off = cell_offset(a.g, q);
+    const double T = a.T[off];
+    const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
+    if (!(T >= a.t_gate && bcv == 0)) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
+      a.Efp[off] = 0.0;
+      if (a.status) a.status[off] = NRG_CELL_GATED;
+    } else {
+      integ = true;
+      double cin[S], c[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) c[s] = cin[s] = a.Y[s][off];
+      const double *Tt = a.table, *dC = a.table + a.rows;
+      for (int i = 0; i + 1 < a.rows; ++i) {
+        const double t0 = Tt[i], t1 = Tt[i + 1];
+        if (T >= t0 && T <= t1) {
+#pragma unroll
+          for (int s = 0; s < S; ++s)
+            c[s] = (dC[s * a.rows + i + 1] * (T - t0)) / (t1 - t0) +
+                   (dC[s * a.rows + i] * (T - t1)) / (t0 - t1);
+        }
+      }
+      if (T < Tt[0]) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) c[s] = 0.0;
+      }
+      if (T > Tt[a.rows - 1]) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) c[s] = dC[s * a.rows + a.rows - 1];
+      }
+      double ef = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        a.Yp[s][off] = (c[s] * W[s]) / a.dt;
+        a.Yp2[s][off] = c[s] - cin[s];
+        ef = ef - (href[s] * c[s]) / a.dt;
+      }
+      a.Efp[off] = ef;
+      if (a.status) a.status[off] = NRG_CELL_OK;
+    }
+  }
+  const unsigned long long n = __popcll(__ballot(integ ? 1 : 0));
+  if ((threadIdx.x & (WAVE - 1)) == 0 && n) atomicAdd(a.acc, n);
+}
+
+// ---------------------------------------------------------------------------
 // Stiffness-ordered queue: bin the inner cells by expected cost, most
 // expensive bin first.  A block bins a contiguous run of BIN_RUN cells (one
 // global atomic per bin and block); within a bin, the cells of a run come in
@@ -953,6 +1033,9 @@ struct nrg_chem_handle {
   long long order_cells = 0;
   bool hist_valid = false;
   GridIdx hist_grid{};
+  // table-approximated mode: T_t[rows], then dC_t[S][rows]
+  double *table = nullptr;
+  int table_rows = 0;
   // step graph (opts.use_graph)
   bool use_graph = false;
   hipGraphExec_t graph_exec = nullptr;
@@ -1241,7 +1324,7 @@ void nrg_chem_destroy(nrg_chem_t *h)
   if (h->ev2) hipEventDestroy(h->ev2);
   if (h->ev3) hipEventDestroy(h->ev3);
   for (void *p : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_acc,
-                  (void *)h->order_list, (void *)h->hist, (void *)h->bins})
+                  (void *)h->order_list, (void *)h->hist, (void *)h->bins, (void *)h->table})
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -1664,6 +1747,73 @@ int nrg_energy_from_temperature_dev(nrg_chem_t *h, const nrg_grid *grid, const d
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
+  return NRG_CHEM_OK;
+}
+
+int nrg_chem_set_table(nrg_chem_t *h, int32_t rows, const double *T_t, const double *dC_t)
+{
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (rows < 2 || !T_t || !dC_t) return set_error(NRG_CHEM_E_ARG, "a table needs >= 2 rows and both arrays");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->table) HIP_TRY(hipFree(h->table));
+  h->table = nullptr;
+  h->table_rows = 0;
+  std::vector<double> t((size_t)(S + 1) * rows);
+  std::memcpy(t.data(), T_t, sizeof(double) * rows);
+  std::memcpy(t.data() + rows, dC_t, sizeof(double) * S * rows);
+  HIP_TRY(hipMalloc(&h->table, t.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(h->table, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  h->table_rows = rows;
+  return NRG_CHEM_OK;
+}
+
+int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                            const double *p, const double *rho, const double *const *Y,
+                            const int8_t *bc, double *const *Y_prod, double *const *Y_prod2,
+                            double *E_f_prod, int32_t *cell_status, nrg_chem_stats *stats)
+{
+  (void)p;
+  (void)rho;
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (!h->table) return set_error(NRG_CHEM_E_ARG, "no table (nrg_chem_set_table)");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return set_error(NRG_CHEM_E_ARG, "time_step must be > 0");
+  if (!T || !Y || !Y_prod || !Y_prod2 || !E_f_prod)
+    return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL");
+  TableArgs a{};
+  int st = make_grid(grid, a.g);
+  if (st) return st;
+  for (int s = 0; s < S; ++s) {
+    if (!Y[s] || !Y_prod[s] || !Y_prod2[s]) return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", s + 1);
+    a.Y[s] = Y[s];
+    a.Yp[s] = Y_prod[s];
+    a.Yp2[s] = Y_prod2[s];
+  }
+  a.dt = dt;
+  a.t_gate = h->k.t_gate;
+  a.mech = h->dev_mech;
+  a.T = T;
+  a.bc = bc;
+  a.Efp = E_f_prod;
+  a.status = cell_status;
+  a.table = h->table;
+  a.rows = h->table_rows;
+  a.acc = h->dev_acc;
+  HIP_TRY(hipSetDevice(h->device));
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
+  unsigned long long acc[16] = {};
+  float ms = 0.f;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  if (a.g.ninner > 0) {
+    const long long nb = (a.g.ninner + 255) / 256;
+    hipLaunchKernelGGL(table_kernel, dim3((unsigned)nb), dim3(256), 0, h->stream, a);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipMemcpyAsync(acc, h->dev_acc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (stats) fill_stats(acc, a.g.ninner, ms, stats);
   return NRG_CHEM_OK;
 }
 

@@ -74,6 +74,7 @@ module nrg_chem_binding
   public :: nrg_chem_create, nrg_chem_destroy, nrg_chem_step, nrg_chem_step_dev
   public :: nrg_chem_closure_dev, nrg_state_equation_dev, nrg_energy_from_temperature_dev
   public :: nrg_chem_stream, nrg_chem_message
+  public :: nrg_chem_set_table, nrg_chem_table_step_dev
 
   interface
     subroutine nrg_chem_default_opts(opts) bind(C, name="nrg_chem_default_opts")
@@ -125,6 +126,30 @@ module nrg_chem_binding
     ! device-resident fields
     function nrg_chem_step_dev(handle, grid, dt, T, p, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, &
                                cell_status, stats) bind(C, name="nrg_chem_step_dev") result(status)
+      import :: c_int, c_ptr, c_double, nrg_grid, nrg_chem_stats
+      type(c_ptr), value :: handle
+      type(nrg_grid), intent(in) :: grid
+      real(c_double), value :: dt
+      type(c_ptr), value :: T, p, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status
+      type(nrg_chem_stats), intent(out) :: stats
+      integer(c_int) :: status
+    end function
+
+    ! read_chemical_kinetics_table (chemical_kinetics_solver.f90:608-648):
+    ! T_t(rows) and dC_t(rows, S) in Fortran order (species-major in C).
+    function nrg_chem_set_table(handle, rows, T_t, dC_t) bind(C, name="nrg_chem_set_table") &
+        result(status)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: handle
+      integer(c_int32_t), value :: rows
+      real(c_double), intent(in) :: T_t(*), dC_t(*)
+      integer(c_int) :: status
+    end function
+
+    ! ODE_solver = 'table_approximated' (chemical_kinetics_solver.f90:250-346)
+    function nrg_chem_table_step_dev(handle, grid, dt, T, p, rho, Y, bc, Y_prod, Y_prod2, &
+                                     E_f_prod, cell_status, stats) &
+        bind(C, name="nrg_chem_table_step_dev") result(status)
       import :: c_int, c_ptr, c_double, nrg_grid, nrg_chem_stats
       type(c_ptr), value :: handle
       type(nrg_grid), intent(in) :: grid

@@ -166,6 +166,30 @@ class ChemistryStep:
         check(self.lib, rc)
         return st
 
+    def set_table(self, T_t, dC_t):
+        """Load a table dC(T) for the table-approximated step: T_t[rows] and
+        dC_t[S, rows] (host arrays; copied to the device)."""
+        T_t = np.ascontiguousarray(T_t, dtype=np.float64)
+        dC_t = np.ascontiguousarray(dC_t, dtype=np.float64)
+        if T_t.ndim != 1 or dC_t.shape != (self.mech.species_number, T_t.shape[0]):
+            raise ValueError("table shapes: T_t (rows,), dC_t (S, rows); got %s, %s"
+                             % (T_t.shape, dC_t.shape))
+        check(self.lib, self.lib.nrg_chem_set_table(self.h, int(T_t.shape[0]),
+                                                    T_t.ctypes.data, dC_t.ctypes.data))
+
+    def table_step_dev(self, grid, dt, T, rho, Y, Y_prod, Y_prod2, E_f_prod, bc=None,
+                       status=None, p=None):
+        """Table-approximated step on device pointers (same fields as step_dev)."""
+        _order_after_torch()
+        st = NrgChemStats()
+        rc = self.lib.nrg_chem_table_step_dev(self.h, C.byref(grid), float(dt), T, p, rho,
+                                              capi.ptr_array(Y), bc, capi.ptr_array(Y_prod),
+                                              capi.ptr_array(Y_prod2), E_f_prod, status,
+                                              C.byref(st))
+        self.last_stats = st
+        check(self.lib, rc)
+        return st
+
     def closure_dev(self, grid, dt, rho, E_f_prod, Y_prod, E_f, Y, T, p=None, bc=None, v=None):
         _order_after_torch()
         vv = capi.ptr_array(v) if v else None
@@ -194,7 +218,7 @@ class ChemistryStep:
 class chemical_kinetics_solver:  # noqa: N801  (reference type name)
     """Mirror of NRG's ``chemical_kinetics_solver`` type (chemical_kinetics_solver.f90:81-103)."""
 
-    def __init__(self, manager, opts=None):
+    def __init__(self, manager, opts=None, ODE_solver="slatec", table_file=None):  # noqa: N803
         self.manager = manager
         self.T = manager.get_cons_field_pointer_by_name("temperature")
         self.p = manager.get_cons_field_pointer_by_name("pressure")
@@ -205,19 +229,53 @@ class chemical_kinetics_solver:  # noqa: N801  (reference type name)
         self.Y_prod = manager.create_vector_field("specie_production_chemistry", "Y_prod_chem", S)
         self.Y_prod2 = manager.create_vector_field("specie_production_chemistry2", "Y_prod_chem2", S)
         self.boundary = manager.bc_markers
-        self.ODE_solver = "slatec"
+        if ODE_solver not in ("slatec", "table_approximated"):
+            raise ValueError("ODE_solver must be 'slatec' or 'table_approximated', not %r"
+                             % (ODE_solver,))
+        self.ODE_solver = ODE_solver
         dev = manager.device.index if manager.device.index is not None else 0
         self.engine = ChemistryStep(manager.chemistry, opts, dev)
         self.cell_status = Field("chemistry_cell_status", "chem_status", manager.grid,
                                  manager.device, dtype=_torch().int32, fill=0)
+        self.T_t = self.Y_t = None
+        if ODE_solver == "table_approximated":                          # :168-170
+            if table_file is None:
+                raise ValueError("ODE_solver='table_approximated' needs table_file")
+            self.read_chemical_kinetics_table(table_file)
 
     def solve_chemical_kinetics(self, time_step):
-        """One chemistry step of length ``time_step`` over all inner cells (:175-358)."""
+        """One chemistry step of length ``time_step`` over all inner cells (:175-358):
+        the SLATEC integration or, with ODE_solver 'table_approximated', the
+        table lookup (:250-346)."""
         g = self.manager.grid
-        return self.engine.step_dev(g, time_step, self.T.ptr(), self.rho.ptr(), self.Y.ptrs(),
-                                    self.Y_prod.ptrs(), self.Y_prod2.ptrs(), self.E_f_prod.ptr(),
-                                    bc=self.boundary.ptr(), status=self.cell_status.ptr(),
-                                    p=self.p.ptr())
+        step = (self.engine.table_step_dev if self.ODE_solver == "table_approximated"
+                else self.engine.step_dev)
+        return step(g, time_step, self.T.ptr(), self.rho.ptr(), self.Y.ptrs(),
+                    self.Y_prod.ptrs(), self.Y_prod2.ptrs(), self.E_f_prod.ptr(),
+                    bc=self.boundary.ptr(), status=self.cell_status.ptr(), p=self.p.ptr())
+
+    def read_chemical_kinetics_table(self, table_file):
+        """Load dC(T) from a table file (:608-648) into the library."""
+        from . import chem_table
+        self.T_t, self.Y_t = chem_table.read_table(table_file,
+                                                   self.manager.chemistry.species_number)
+        self.engine.set_table(self.T_t, self.Y_t)
+
+    def write_chemical_kinetics_table(self, table_file):
+        """Write T and Y_prod2 of the grid line j = k = 1 (:542-606), from the first
+        cell with 300.5 K < T < 310 K (Fortran index 0 if none) to the hottest."""
+        from . import chem_table
+        g = self.manager.grid
+        j, k = 1 - g.lo[1], 1 - g.lo[2]
+        i0 = g.inner_lo[0] - g.lo[0]                        # array index of the first inner cell
+        T_line = self.T.data[k, j, :].cpu().numpy()
+        lo, hi = chem_table.table_rows(T_line[i0:g.inner_hi[0] - g.lo[0] + 1],
+                                       none_index=-g.inner_lo[0])
+        lo, hi = max(i0 + lo, 0), i0 + hi
+        dC = np.stack([f.data[k, j, lo:hi + 1].cpu().numpy() for f in self.Y_prod2.pr])
+        chem_table.write_table(table_file, T_line[lo:hi + 1], dC,
+                               self.manager.chemistry.species_names)
+        return lo, hi
 
     def autotune(self, time_step):
         """Run the library's autotuner (nrg_chem_autotune_dev) on the current
@@ -244,6 +302,6 @@ class chemical_kinetics_solver:  # noqa: N801  (reference type name)
                                 p=self.p.ptr(), bc=self.boundary.ptr())
 
 
-def chemical_kinetics_solver_c(manager, opts=None):
+def chemical_kinetics_solver_c(manager, opts=None, ODE_solver="slatec", table_file=None):  # noqa: N803
     """Constructor with the reference's name (chemical_kinetics_solver.f90:105-111)."""
-    return chemical_kinetics_solver(manager, opts)
+    return chemical_kinetics_solver(manager, opts, ODE_solver, table_file)

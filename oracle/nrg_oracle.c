@@ -1033,6 +1033,56 @@ int or_chem_cells(const nrg_chem_mech *m, const nrg_chem_opts *o, int64_t ncells
   return rc;
 }
 
+/* interpolate_chemical_kinetics (cm/chemical_kinetics_solver.f90:650-685): every
+ * bracketing interval overwrites c (the last one wins), then the two range
+ * rules.  Fortran evaluates a*b/c left to right. */
+static void or_table_interpolate(int S, int32_t rows, const double *T_t, const double *dC_t,
+                                 double T, double *c)
+{
+  for (int32_t i = 0; i + 1 < rows; i++) {
+    if (T >= T_t[i] && T <= T_t[i + 1]) {
+      for (int s = 0; s < S; s++)
+        c[s] = (dC_t[s * rows + i + 1] * (T - T_t[i])) / (T_t[i + 1] - T_t[i]) +
+               (dC_t[s * rows + i] * (T - T_t[i + 1])) / (T_t[i] - T_t[i + 1]);
+    }
+  }
+  if (T < T_t[0])
+    for (int s = 0; s < S; s++) c[s] = 0.0;
+  if (T > T_t[rows - 1])
+    for (int s = 0; s < S; s++) c[s] = dC_t[s * rows + rows - 1];
+}
+
+void or_table_cells(const nrg_chem_mech *m, int64_t ncells, double dt, double t_gate,
+                    int32_t rows, const double *T_t, const double *dC_t,
+                    const double *T, const double *const *Y, const int8_t *bc,
+                    double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                    int32_t *cell_status)
+{
+  const int S = m->species_number;
+  double href[OR_NMAX];
+  for (int s = 0; s < S; s++) href[s] = or_h_molar(m, T_REF, s);     /* :331 */
+  for (int64_t q = 0; q < ncells; q++) {
+    if (!(T[q] >= t_gate && (bc ? bc[q] : 0) == 0)) {                  /* :228-231, :341-346 */
+      for (int s = 0; s < S; s++) Y_prod[s][q] = 0.0;
+      E_f_prod[q] = 0.0;
+      if (cell_status) cell_status[q] = NRG_CELL_GATED;
+      continue;
+    }
+    double cin[OR_NMAX], c[OR_NMAX];
+    for (int s = 0; s < S; s++) c[s] = cin[s] = Y[s][q];              /* :255-262 */
+    or_table_interpolate(S, rows, T_t, dC_t, T[q], c);                 /* :314-315 */
+    double ef = 0.0;                                                   /* :235 */
+    for (int s = 0; s < S; s++) {
+      const double W = m->molar_masses[s];
+      Y_prod2[s][q] = c[s] - cin[s];                                   /* :321 */
+      Y_prod[s][q] = (c[s] * W) / dt;                                  /* :323-325 */
+      ef = ef - (href[s] * c[s]) / dt;                                 /* :334-335 */
+    }
+    E_f_prod[q] = ef;
+    if (cell_status) cell_status[q] = NRG_CELL_OK;
+  }
+}
+
 void or_closure_cells(const nrg_chem_mech *m, int64_t ncells, double dt,
                       const double *rho, const double *E_f_prod, const double *const *Y_prod,
                       const int8_t *bc, double *E_f, double *const *Y, double *T, double *p,

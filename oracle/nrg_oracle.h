@@ -140,6 +140,17 @@ void or_closure_cells(const nrg_chem_mech *m, int64_t ncells, double dt,
 void or_energy_from_temperature_cells(const nrg_chem_mech *m, int64_t ncells, const double *T,
                                       const double *const *Y, double *E_f);
 
+/* Table-approximated mode (cm/chemical_kinetics_solver.f90:250-346 with
+ * ODE_solver = 'table_approximated'; interpolate_chemical_kinetics :650-685)
+ * over a flat list of cells.  Table: T_t[rows], dC_t[s*rows + i] (Y_t(s+1, i+1)).
+ * Gate as the SLATEC path (T >= t_gate and bc == 0); gated cells get
+ * Y_prod = 0, E_f_prod = 0, Y_prod2 untouched. */
+void or_table_cells(const nrg_chem_mech *m, int64_t ncells, double dt, double t_gate,
+                    int32_t rows, const double *T_t, const double *dC_t,
+                    const double *T, const double *const *Y, const int8_t *bc,
+                    double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                    int32_t *cell_status);
+
 void or_default_opts(nrg_chem_opts *o);
 
 #ifdef __cplusplus

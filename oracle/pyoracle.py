@@ -85,6 +85,9 @@ class Oracle:
         L.or_closure_cells.argtypes = [C.c_void_p, C.c_int64, C.c_double, _dp, _dp, C.c_void_p,
                                        C.c_void_p, _dp, C.c_void_p, _dp, C.c_void_p, C.c_int]
         L.or_energy_from_temperature_cells.argtypes = [C.c_void_p, C.c_int64, _dp, C.c_void_p, _dp]
+        L.or_table_cells.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_double, C.c_int32, _dp,
+                                     _dp, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _dp,
+                                     C.c_void_p]
         L.or_chem_ctx_size.restype = C.c_size_t
         L.or_chem_ctx_init.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         self.chem_fun = RHS_FN(C.cast(L.or_chem_fun, C.c_void_p).value)
@@ -219,6 +222,27 @@ class Oracle:
                                     nj.ctypes.data, nthreads)
         return dict(rc=rc, Y_prod=yp, Y_prod2=yp2, E_f_prod=ef, status=st, nstep=ns, nfe=nf,
                     nje=nj)
+
+    def table_cells(self, dt, T_t, dC_t, T, Y, bc=None, Y_prod2=None, t_gate=305.0):
+        """Table-approximated step over a flat list of cells.  dC_t: (S, rows)."""
+        n = len(T)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        T_t = np.ascontiguousarray(T_t, dtype=np.float64)
+        dC_t = np.ascontiguousarray(dC_t, dtype=np.float64)
+        yp = np.zeros((self.S, n))
+        yp2 = (np.zeros((self.S, n)) if Y_prod2 is None
+               else np.array(Y_prod2, dtype=np.float64, order="C"))
+        ef = np.zeros(n)
+        st = np.zeros(n, dtype=np.int32)
+        bcp = None
+        if bc is not None:
+            bc = np.ascontiguousarray(bc, dtype=np.int8)
+            bcp = bc.ctypes.data
+        self.lib.or_table_cells(self.mp, n, dt, t_gate, len(T_t), _as_dp(T_t), _as_dp(dC_t),
+                                _as_dp(T), _ptrs(list(Y)), bcp, _ptrs(list(yp)), _ptrs(list(yp2)),
+                                _as_dp(ef), st.ctypes.data)
+        return dict(Y_prod=yp, Y_prod2=yp2, E_f_prod=ef, status=st)
 
     def closure_cells(self, dt, rho, E_f_prod, Y_prod, E_f, Y, T, bc=None, nthreads=1):
         """Split closure + e->T.  Returns (E_f, Y, T, p) new arrays."""
