@@ -63,6 +63,10 @@ def parse():
     ap.add_argument("--gate-dt-max", type=float, default=None, help="K")
     ap.add_argument("--gate-dt-freeze", type=float, default=None, help="K")
     ap.add_argument("--gate-dy-freeze", type=float, default=None)
+    ap.add_argument("--table", type=int, default=1,
+                    help="time the table_approximated mode beside the FP64 integrator (0: skip)")
+    ap.add_argument("--table-line", type=int, default=512,
+                    help="cells of the flame line the table is written from")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this size before the timed region, as the "
                          "known-size reference for HBM counters (FETCH_SIZE / WRITE_SIZE)")
@@ -242,6 +246,10 @@ def main():
             audit = {"cells": m, "max_abs_Y": float(dY), "max_rel_T": relT,
                      "bar_Y": 1e-9, "bar_T": 1e-10}
 
+    # ---- the FP64 run's final state, against which the other legs are audited ----------
+    T_ref = dm.fields["temperature"].inner(g).clone()
+    Y_ref = [Yf.pr[s].inner(g).clone() for s in range(S)]
+
     # ---- surrogate + FP64 fallback from the same initial state ---------------------------
     sur = None
     path = args.surrogate
@@ -249,7 +257,10 @@ def main():
         path = os.path.join(ROOT, "nrg_amd", "data", "surrogate_keromnes.npz")
         path = path if os.path.exists(path) else "none"
     if path != "none":
-        sur = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, value, S)
+        sur = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S)
+
+    # ---- table-approximated mode from the same initial state -----------------------------
+    tab = table_leg(args, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S) if args.table else None
 
     line = {
         "metric": METRIC,
@@ -277,11 +288,15 @@ def main():
         "work": {"nfe_per_step": nfe / args.steps, "integrated_cells_per_step": integ / args.steps,
                  "nstep_max": nstep_max},
         "surrogate": sur,
+        "table_mode": tab,
     }
     print(json.dumps(line), flush=True)
 
 
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_HBM_GBS = 8000.0
+TABLE_BYTES_INTEGRATED = 8 + 1 + 9 * 8 + (9 + 9) * 8 + 8 + 4
+TABLE_BYTES_GATED = 8 + 1 + 9 * 8 + 8 + 4
 HBM_PROFILE = os.path.join(ROOT, "profiles", "chem_step_256cube_hbm_bytes.txt")
 
 
@@ -314,7 +329,94 @@ def profiled_traffic(cells_per_call, kernels=("chem_bdf_kernel", "rates_kernel")
             "source": os.path.relpath(HBM_PROFILE, ROOT)}
 
 
-def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, fp64_value, S):
+def restore_state(dm, fields0, Y0, S):
+    for k, v in fields0.items():
+        dm.fields[k].data.copy_(v)
+    Yf = dm.fields["specie_mass_fraction"]
+    for s in range(S):
+        Yf.pr[s].data.copy_(Y0[s])
+
+
+def audit_against(dm, T_ref, Y_ref, S, steps):
+    """Final state of a leg against the FP64 run's final state (all inner cells)."""
+    import torch
+    g = dm.grid
+    Yf = dm.fields["specie_mass_fraction"]
+    T_s = dm.fields["temperature"].inner(g)
+    relT = ((T_s - T_ref).abs() / T_ref).reshape(-1)
+    major = (0, 1, 6)       # H2, O2, H2O
+    dY = torch.stack([(Yf.pr[s].inner(g) - Y_ref[s]).abs().reshape(-1) for s in major])
+    dY_all = torch.stack([(Yf.pr[s].inner(g) - Y_ref[s]).abs().reshape(-1) for s in range(S)])
+    q = lambda t, p: float(torch.quantile(t[::max(1, t.numel() // 4_000_000)].float(), p))  # noqa: E731
+    return {
+        "steps": steps, "cells": int(T_s.numel()),
+        "max_rel_T": float(relT.max()), "p99_rel_T": q(relT, 0.99),
+        "max_abs_Y_major": float(dY.max()), "p99_abs_Y_major": q(dY.max(dim=0).values, 0.99),
+        "max_abs_Y_any": float(dY_all.max()),
+        "bar_rel_T": 1e-2, "bar_abs_Y_major": 1e-3,
+    }
+
+
+def table_leg(args, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S):
+    """NRG's table_approximated mode as a non-learned baseline: a table dC(T)
+    written from this library's FP64 step on a 1-D 15 % H2-air flame line at
+    1 atm (the reference's table is named for that mixture), then the table
+    split step (lookup + closure) over the same steps from the same initial
+    state, audited against the FP64 run."""
+    import numpy as np
+    import torch
+
+    from nrg_amd import chem_table as ct
+    e = solver.engine
+    T_t, dC_t = ct.build_table(e, mech, args.dt, n=args.table_line)
+    e.set_table(T_t, dC_t)
+    restore_state(dm, fields0, Y0, S)
+    mode0 = solver.ODE_solver
+    solver.ODE_solver = "table_approximated"
+
+    def step():
+        st = solver.solve_chemical_kinetics(args.dt)
+        solver.apply_split_sources(args.dt)
+        return st
+
+    try:
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        recs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            recs.append(step())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        solver.ODE_solver = mode0
+    ncells = dm.inner_cells()
+    value = ncells * args.steps / wall
+    k_ms = float(np.mean([r.kernel_ms for r in recs]))
+    integ = float(np.mean([r.cells_integrated for r in recs]))
+    # table_kernel's algorithmic bytes: an integrated cell reads T, marker and 9 Y
+    # (81 B) and writes 9 Y_prod, 9 Y_prod2, E_f_prod and its status (156 B); a
+    # gated one reads T and marker and writes 9 Y_prod, E_f_prod, status (93 B)
+    nbytes = integ * TABLE_BYTES_INTEGRATED + (ncells - integ) * TABLE_BYTES_GATED
+    gbs = nbytes / (k_ms * 1e-3) / 1e9
+    return {
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "speedup_vs_fp64": round(value / fp64_value, 3),
+        "dtype": "f64",
+        "table_rows": int(T_t.size), "table_T_range": [float(T_t[0]), float(T_t[-1])],
+        "table_source": "FP64 step on a %d-cell 15%% H2-air flame line at 1 atm" % args.table_line,
+        "kernel_ms": round(k_ms, 4),
+        "roofline": {"kernel": "table_kernel", "bound": "hbm", "achieved": round(gbs, 1),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                     "algorithmic_bytes_per_launch": nbytes},
+        "audit_vs_fp64": audit_against(dm, T_ref, Y_ref, S, args.warmup + args.steps),
+    }
+
+
+def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S):
     """Time the learned split step (network + gate + FP64 fallback + closure)
     over the same number of steps from the same initial state, and audit its
     final state against the FP64 integrator's."""
@@ -322,16 +424,7 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, fp64_value, S):
     import torch
 
     from nrg_amd import surrogate as sg
-    g = dm.grid
-    Yf = dm.fields["specie_mass_fraction"]
-    # the FP64 run's final state
-    T_ref = dm.fields["temperature"].inner(g).clone()
-    Y_ref = [Yf.pr[s].inner(g).clone() for s in range(S)]
-    # back to the initial state
-    for k, v in fields0.items():
-        dm.fields[k].data.copy_(v)
-    for s in range(S):
-        Yf.pr[s].data.copy_(Y0[s])
+    restore_state(dm, fields0, Y0, S)
     z = np.load(path, allow_pickle=False)
     mlp, norm = sg.load(path)
     gkw = {}
@@ -371,13 +464,6 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, fp64_value, S):
     infer_ms = mean(lambda r: r.infer_ms)
     flops = 2 * (12 * 256 + 3 * 256 * 256 + 256 * 10) * ncells      # 404,480 per inferred cell
     tflops = flops / (infer_ms * 1e-3) / 1e12
-    # audit: final state against the FP64 run's final state (all inner cells)
-    T_s = dm.fields["temperature"].inner(g)
-    relT = ((T_s - T_ref).abs() / T_ref).reshape(-1)
-    major = (0, 1, 6)       # H2, O2, H2O
-    dY = torch.stack([(Yf.pr[s].inner(g) - Y_ref[s]).abs().reshape(-1) for s in major])
-    dY_all = torch.stack([(Yf.pr[s].inner(g) - Y_ref[s]).abs().reshape(-1) for s in range(S)])
-    q = lambda t, p: float(torch.quantile(t[::max(1, t.numel() // 4_000_000)].float(), p))
     res = {
         "value": round(value, 1),
         "unit": "cell-steps/s",
@@ -398,13 +484,7 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, fp64_value, S):
         "roofline_infer": {"bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_BF16_TFLOPS,
                            "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 4),
                            "kernel": "featurize_kernel + mlp_infer_kernel (404,480 FLOP/cell)"},
-        "audit_vs_fp64": {
-            "steps": args.warmup + args.steps, "cells": ncells,
-            "max_rel_T": float(relT.max()), "p99_rel_T": q(relT, 0.99),
-            "max_abs_Y_major": float(dY.max()), "p99_abs_Y_major": q(dY.max(dim=0).values, 0.99),
-            "max_abs_Y_any": float(dY_all.max()),
-            "bar_rel_T": 1e-2, "bar_abs_Y_major": 1e-3,
-        },
+        "audit_vs_fp64": audit_against(dm, T_ref, Y_ref, S, args.warmup + args.steps),
         "model": os.path.relpath(path, ROOT),
     }
     mlp.close()

@@ -46,7 +46,20 @@ struct GridIdx {
   int ilo0, ilo1, ilo2;       // inner lower bounds
   int ix, iy, iz;             // inner extents
   long long ninner;
+  long long pad0, pad1;       // allocation offsets [pad0, pad1) holding every inner cell; pad0 % 64 == 0
 };
+
+// Is allocation offset p an inner cell (not a ghost)?
+NRG_DI bool inner_at(const GridIdx &g, long long p)
+{
+  const long long r = p / g.nx;
+  const long long i = p - r * g.nx;
+  const long long j = r % g.ny;
+  const long long k = r / g.ny;
+  return (unsigned long long)(i - (g.ilo0 - g.lo0)) < (unsigned long long)g.ix &&
+         (unsigned long long)(j - (g.ilo1 - g.lo1)) < (unsigned long long)g.iy &&
+         (unsigned long long)(k - (g.ilo2 - g.lo2)) < (unsigned long long)g.iz;
+}
 
 NRG_DI long long cell_offset(const GridIdx &g, long long q)
 {
@@ -539,21 +552,65 @@ struct TableArgs {
   int32_t *status;
   const double *table;      // T_t[rows], dC_t[S][rows]
   int rows;
+  int ascending;            // T_t strictly increasing: bisection finds the same interval
   unsigned long long *acc;  // [0] integrated cells
 };
 
-__global__ void __launch_bounds__(256) table_kernel(TableArgs a)
+// The interval the reference's scan ends on (the last i with
+// T_t[i] <= T <= T_t[i+1]), or -1 if none.  For a strictly increasing table
+// that is the largest i with T_t[i] <= T, capped at rows-2 (T on a node k
+// sits in intervals k-1 and k; k wins), so bisection gives it in log2(rows)
+// probes.  Any other table is scanned in full.
+__device__ __forceinline__ int table_interval(const double *Tt, int rows, int ascending, double T)
 {
+  if (ascending) {
+    if (!(T >= Tt[0] && T <= Tt[rows - 1])) return -1;
+    int lo = 0, hi = rows - 1;      // Tt[lo] <= T; the answer lies in [lo, hi]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (Tt[mid] <= T) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo < rows - 1 ? lo : rows - 2;
+  }
+  int last = -1;
+  for (int i = 0; i + 1 < rows; ++i)
+    if (T >= Tt[i] && T <= Tt[i + 1]) last = i;
+  return last;
+}
+
+constexpr int TABLE_THREADS = 256;
+constexpr int TABLE_BLOCKS_PER_CU = 4;        // four 4-wave blocks: one wave per SIMD each
+constexpr int TABLE_LDS_BYTES = 32768;        // a table up to this size is staged in LDS
+
+// A grid of TABLE_BLOCKS_PER_CU blocks per CU strides over the cells, so the
+// table is staged into LDS once per block (not once per 256 cells) and the
+// integrated-cell count takes one global atomic per block.
+template <bool TAB_LDS>
+__global__ void __launch_bounds__(TABLE_THREADS) table_kernel(TableArgs a)
+{
+  extern __shared__ double tab_lds[];         // T_t[rows], dC_t[S][rows] when TAB_LDS
   __shared__ double W[S], href[S];
+  __shared__ unsigned long long n_integ;
   if (threadIdx.x < S) {
     W[threadIdx.x] = a.mech->W[threadIdx.x];
     href[threadIdx.x] = a.mech->h_ref[threadIdx.x];
   }
+  if (threadIdx.x == 0) n_integ = 0;
+  if (TAB_LDS)
+    for (int k = threadIdx.x; k < (S + 1) * a.rows; k += TABLE_THREADS) tab_lds[k] = a.table[k];
   __syncthreads();
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool integ = false;
-  if (q < a.g.ninner) {
-    const long long off = cell_offset(a.g, q);
+  const double *Tt = TAB_LDS ? tab_lds : a.table;
+  const double *dC = Tt + a.rows;
+  unsigned long long cnt = 0;
+  const long long stride = (long long)gridDim.x * TABLE_THREADS;
+  // Waves walk the allocation in 64-cell slices aligned with it (from the
+  // first inner cell's slice to the last inner cell) and skip ghost cells:
+  // with ghost layers a row of inner cells starts off a cache line, and
+  // walking inner cells would make every wave touch one line more per field.
+  for (long long off = a.g.pad0 + (long long)blockIdx.x * TABLE_THREADS + threadIdx.x;
+       off < a.g.pad1; off += stride) {
+    if (!inner_at(a.g, off)) continue;
     const double T = a.T[off];
     const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
     if (!(T >= a.t_gate && bcv == 0)) {
@@ -561,42 +618,41 @@ __global__ void __launch_bounds__(256) table_kernel(TableArgs a)
       for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
       a.Efp[off] = 0.0;
       if (a.status) a.status[off] = NRG_CELL_GATED;
-    } else {
-      integ = true;
-      double cin[S], c[S];
-#pragma unroll
-      for (int s = 0; s < S; ++s) c[s] = cin[s] = a.Y[s][off];
-      const double *Tt = a.table, *dC = a.table + a.rows;
-      for (int i = 0; i + 1 < a.rows; ++i) {
-        const double t0 = Tt[i], t1 = Tt[i + 1];
-        if (T >= t0 && T <= t1) {
-#pragma unroll
-          for (int s = 0; s < S; ++s)
-            c[s] = (dC[s * a.rows + i + 1] * (T - t0)) / (t1 - t0) +
-                   (dC[s * a.rows + i] * (T - t1)) / (t0 - t1);
-        }
-      }
-      if (T < Tt[0]) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) c[s] = 0.0;
-      }
-      if (T > Tt[a.rows - 1]) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) c[s] = dC[s * a.rows + a.rows - 1];
-      }
-      double ef = 0.0;
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        a.Yp[s][off] = (c[s] * W[s]) / a.dt;
-        a.Yp2[s][off] = c[s] - cin[s];
-        ef = ef - (href[s] * c[s]) / a.dt;
-      }
-      a.Efp[off] = ef;
-      if (a.status) a.status[off] = NRG_CELL_OK;
+      continue;
     }
+    ++cnt;
+    double cin[S], c[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) c[s] = cin[s] = a.Y[s][off];
+    const int i = table_interval(Tt, a.rows, a.ascending, T);
+    if (i >= 0) {
+      const double t0 = Tt[i], t1 = Tt[i + 1];
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        c[s] = (dC[s * a.rows + i + 1] * (T - t0)) / (t1 - t0) +
+               (dC[s * a.rows + i] * (T - t1)) / (t0 - t1);
+    }
+    if (T < Tt[0]) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) c[s] = 0.0;
+    }
+    if (T > Tt[a.rows - 1]) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) c[s] = dC[s * a.rows + a.rows - 1];
+    }
+    double ef = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      a.Yp[s][off] = (c[s] * W[s]) / a.dt;
+      a.Yp2[s][off] = c[s] - cin[s];
+      ef = ef - (href[s] * c[s]) / a.dt;
+    }
+    a.Efp[off] = ef;
+    if (a.status) a.status[off] = NRG_CELL_OK;
   }
-  const unsigned long long n = __popcll(__ballot(integ ? 1 : 0));
-  if ((threadIdx.x & (WAVE - 1)) == 0 && n) atomicAdd(a.acc, n);
+  if (cnt) atomicAdd(&n_integ, cnt);
+  __syncthreads();
+  if (threadIdx.x == 0 && n_integ) atomicAdd(a.acc, n_integ);
 }
 
 // ---------------------------------------------------------------------------
@@ -1036,6 +1092,8 @@ struct nrg_chem_handle {
   // table-approximated mode: T_t[rows], then dC_t[S][rows]
   double *table = nullptr;
   int table_rows = 0;
+  int table_ascending = 0;
+  int num_cu = 1;
   // step graph (opts.use_graph)
   bool use_graph = false;
   hipGraphExec_t graph_exec = nullptr;
@@ -1174,6 +1232,14 @@ int make_grid(const nrg_grid *grid, GridIdx &g)
   g.iy = std::max(0, grid->inner_hi[1] - grid->inner_lo[1] + 1);
   g.iz = std::max(0, grid->inner_hi[2] - grid->inner_lo[2] + 1);
   g.ninner = (long long)g.ix * g.iy * g.iz;
+  g.pad0 = g.pad1 = 0;
+  if (g.ninner > 0) {
+    auto off = [&](int i, int j, int k) {
+      return (long long)(i - g.lo0) + (long long)g.nx * ((j - g.lo1) + (long long)g.ny * (k - g.lo2));
+    };
+    g.pad0 = off(grid->inner_lo[0], grid->inner_lo[1], grid->inner_lo[2]) & ~63ll;
+    g.pad1 = off(grid->inner_hi[0], grid->inner_hi[1], grid->inner_hi[2]) + 1;
+  }
   return NRG_CHEM_OK;
 }
 
@@ -1298,6 +1364,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     if (o.waves_per_cu > 0) per_cu = o.waves_per_cu;
     if (per_cu < 1) per_cu = 1;
     h->persistent_blocks = per_cu * prop.multiProcessorCount;
+    h->num_cu = std::max(prop.multiProcessorCount, 1);
   }
   if (hipMemcpy(h->dev_mech, &h->host_mech, sizeof(MechCoef), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipMemcpy mech"));
@@ -1765,6 +1832,9 @@ int nrg_chem_set_table(nrg_chem_t *h, int32_t rows, const double *T_t, const dou
   HIP_TRY(hipMalloc(&h->table, t.size() * sizeof(double)));
   HIP_TRY(hipMemcpy(h->table, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
   h->table_rows = rows;
+  h->table_ascending = 1;
+  for (int32_t i = 0; i + 1 < rows; ++i)
+    if (!(T_t[i] < T_t[i + 1])) h->table_ascending = 0;
   return NRG_CHEM_OK;
 }
 
@@ -1798,6 +1868,7 @@ int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, cons
   a.status = cell_status;
   a.table = h->table;
   a.rows = h->table_rows;
+  a.ascending = h->table_ascending;
   a.acc = h->dev_acc;
   HIP_TRY(hipSetDevice(h->device));
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
@@ -1805,8 +1876,14 @@ int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, cons
   float ms = 0.f;
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   if (a.g.ninner > 0) {
-    const long long nb = (a.g.ninner + 255) / 256;
-    hipLaunchKernelGGL(table_kernel, dim3((unsigned)nb), dim3(256), 0, h->stream, a);
+    const long long nb = std::min<long long>((a.g.pad1 - a.g.pad0 + TABLE_THREADS - 1) / TABLE_THREADS,
+                                             (long long)TABLE_BLOCKS_PER_CU * h->num_cu);
+    const size_t tab_bytes = sizeof(double) * (size_t)(S + 1) * (size_t)a.rows;
+    if (tab_bytes <= (size_t)TABLE_LDS_BYTES)
+      hipLaunchKernelGGL(table_kernel<true>, dim3((unsigned)nb), dim3(TABLE_THREADS), tab_bytes,
+                         h->stream, a);
+    else
+      hipLaunchKernelGGL(table_kernel<false>, dim3((unsigned)nb), dim3(TABLE_THREADS), 0, h->stream, a);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(h->ev1, h->stream));

@@ -9,8 +9,8 @@ prints, per kernel, the mean duration of its last N dispatches (the timed
 steps, after warm-up) from the per-dispatch trace.  ``counter``
 prints, per kernel, the average per-dispatch value of each collected counter
 (FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  With ``--calib-mib`` the
-largest device copy kernel (the known-size copy bench.py runs with
---copy-calibration-mib) is taken as the reference: its expected bytes divided
+largest dispatch of torch's direct device copy (the known-size copy bench.py
+runs with --copy-calibration-mib) is taken as the reference: its expected bytes divided
 by its measured bytes gives the factor applied to the other kernels.  Each
 calibrated byte counter also gets a line
     TRAFFIC <tab> kernel <tab> counter <tab> bytes per dispatch
@@ -64,7 +64,9 @@ def counters(path, calib_mib=None):
         disp[k].add(r.get("Dispatch_Id") or r.get("Dispatch-Id"))
     factor = {}
     if calib_mib:
-        copies = [k for k in acc if "copy" in k.lower() or "elementwise" in k.lower()]
+        # torch's contiguous device copy (bench.py's known-size copy); concatenation
+        # and other copy-like kernels of the run are not calibration candidates
+        copies = [k for k in acc if "direct_copy" in k]
         best = None
         for k in copies:
             for c, v in acc[k].items():

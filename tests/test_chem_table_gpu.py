@@ -27,8 +27,11 @@ def _tables():
     # a table that ends below its start: both range rules fire between the ends
     T_down = np.array([2400.0, 1500.0, 600.0])
     dC_down = rng.standard_normal((S, T_down.size)) * 1e-5
+    # more rows than the kernel stages in LDS (32 KB): read from global memory
+    T_long = np.sort(rng.uniform(300.0, 2700.0, 600))
+    dC_long = rng.standard_normal((S, T_long.size)) * 1e-4
     return {"ascending": (T_up, dC_up), "overlapping": (T_mix, dC_mix),
-            "descending": (T_down, dC_down)}
+            "descending": (T_down, dC_down), "long": (T_long, dC_long)}
 
 
 TABLES = _tables()
@@ -38,7 +41,13 @@ def _state(mech, n=5000, seed=4):
     st = inputs.to_numpy(inputs.ignition_mix(mech, n, seed=seed, frac_window=0.5))
     rng = np.random.default_rng(seed)
     T = rng.uniform(200.0, 2900.0, n)         # below, inside and above every table
-    T[:5] = [305.0, 304.999, 320.0, 2600.0, 1100.0]     # the gate edge and table nodes
+    # the gate edge, the table ends and interior nodes (a node sits in two
+    # intervals; the later one wins)
+    T[:9] = [305.0, 304.999, 320.0, 2600.0, 1100.0, 900.0, 800.0, 1500.0, 2400.0]
+    T_up = TABLES["ascending"][0]
+    if n >= 200:
+        T[9:9 + T_up.size] = T_up
+        T[100:100 + T_up.size - 1] = np.nextafter(T_up[1:], 0.0)  # just below each node
     bc = np.zeros(n, dtype=np.int8)
     bc[7:40:3] = 1
     return T, st["rho"], np.ascontiguousarray(st["Y"]), bc

@@ -14,8 +14,11 @@ def test_counter_summary_calibrates_and_emits_traffic(tmp_path):
     from nrg_amd import rocprof_summary as rs
     p = tmp_path / "c.csv"
     rows = [
-        # a 512 MiB copy measured at 2.25x its bytes, and two chemistry dispatches
-        ("void at::native::elementwise_kernel<copy>", "FETCH_SIZE", 1, 512 * 1024 * 2.25),
+        # a 512 MiB copy measured at 2.25x its bytes, a larger concatenation (not a
+        # calibration candidate) and two chemistry dispatches
+        ("void at::native::elementwise_kernel_manual_unroll<128, 4, at::native::direct_copy_kernel_cuda>",
+         "FETCH_SIZE", 1, 512 * 1024 * 2.25),
+        ("void at::native::CatArrayBatchedCopy_contig<double>", "FETCH_SIZE", 4, 4096 * 1024.0),
         ("void (anonymous namespace)::chem_bdf_kernel<true, false>((anonymous namespace)::StepArgs)",
          "FETCH_SIZE", 2, 900.0),
         ("void (anonymous namespace)::chem_bdf_kernel<true, false>((anonymous namespace)::StepArgs)",
