@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--gate-dy-freeze", type=float, default=None)
     ap.add_argument("--table", type=int, default=1,
                     help="time the table_approximated mode beside the FP64 integrator (0: skip)")
+    ap.add_argument("--low-mach", type=int, default=1,
+                    help="time the low-Mach coupling kernels on the FP64 run's outputs (0: skip)")
     ap.add_argument("--table-line", type=int, default=512,
                     help="cells of the flame line the table is written from")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -259,6 +261,9 @@ def main():
     if path != "none":
         sur = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S)
 
+    # ---- low-Mach (FDS) coupling kernels on the FP64 run's outputs ------------------------
+    lm = low_mach_leg(args, dm, solver, S) if args.low_mach else None
+
     # ---- table-approximated mode from the same initial state -----------------------------
     tab = table_leg(args, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S) if args.table else None
 
@@ -289,6 +294,7 @@ def main():
                  "nstep_max": nstep_max},
         "surrogate": sur,
         "table_mode": tab,
+        "low_mach": lm,
     }
     print(json.dumps(line), flush=True)
 
@@ -355,6 +361,71 @@ def audit_against(dm, T_ref, Y_ref, S, steps):
         "max_abs_Y_any": float(dY_all.max()),
         "bar_rel_T": 1e-2, "bar_abs_Y_major": 1e-3,
     }
+
+
+# algorithmic bytes per inner cell of the low-Mach coupling kernels as bench.py
+# calls them (no markers set, three velocity components, all optional outputs):
+# EOS predictor reads rho, 9 Y, dp_stat_dt, p_stat, 3 v, marker (121 B) and writes
+# 11 fields (88 B); the species predictor with the chemistry source reads rho,
+# 9 Y, 9 Y_prod, marker (153 B) and writes rho_int, 9 Y_int (80 B); the reactive
+# divergence reads rho, T, 9 Y, W_mix, E_f_prod, 9 Y_prod, marker (177 B) and
+# writes div_v (8 B)
+LOW_MACH_BYTES = {"low_mach_eos_kernel": 121 + 88, "low_mach_species_kernel": 153 + 80,
+                  "low_mach_div_kernel": 177 + 8}
+
+
+def low_mach_leg(args, dm, solver, S, reps=10):
+    """Time the three low-Mach coupling kernels (include/nrg_low_mach.h) on
+    the 256^3 fields the FP64 run left (its Y_prod and E_f_prod are the
+    chemistry sources), each with HIP events on the library's stream."""
+    import numpy as np
+    import torch
+    e = solver.engine
+    g = dm.grid
+    stream = torch.cuda.ExternalStream(e.lib.nrg_chem_stream(e.h))
+    shp = dm.fields["temperature"].data.shape
+    new = lambda fill=0.0: torch.full(shp, fill, dtype=torch.float64, device="cuda:0")  # noqa: E731
+    rho = dm.fields["density"]
+    Yf = dm.fields["specie_mass_fraction"]
+    p_stat = dm.fields["pressure"].data.clone()
+    p_old, dp = new(), new(1.0e3)
+    T, mmm, p, cp, gam, hs, vs, ei, Ef = (new() for _ in range(9))
+    v = [new(1.0), new(2.0), new(3.0)]
+    div, rho_i = new(), new()
+    Y_i = [new() for _ in range(S)]
+    ptr = lambda L: [x.data_ptr() for x in L]  # noqa: E731
+    calls = {
+        "low_mach_eos_kernel": lambda: e.low_mach_state_equation_dev(
+            g, args.dt, True, rho.ptr(), Yf.ptrs(), dp.data_ptr(), p_stat.data_ptr(),
+            p_old.data_ptr(), T.data_ptr(), mmm.data_ptr(), v=ptr(v), bc=dm.bc_markers.ptr(),
+            p=p.data_ptr(), mixture_cp=cp.data_ptr(), gamma=gam.data_ptr(), h_s=hs.data_ptr(),
+            v_s=vs.data_ptr(), e_i=ei.data_ptr(), E_f=Ef.data_ptr()),
+        "low_mach_div_kernel": lambda: e.low_mach_chem_divergence_dev(
+            g, rho.ptr(), T.data_ptr(), Yf.ptrs(), mmm.data_ptr(), solver.E_f_prod.ptr(),
+            solver.Y_prod.ptrs(), div.data_ptr(), bc=dm.bc_markers.ptr()),
+        "low_mach_species_kernel": lambda: e.low_mach_species_dev(
+            g, args.dt, False, rho.ptr(), Yf.ptrs(), rho_i.data_ptr(), ptr(Y_i),
+            Y_prod_chem=solver.Y_prod.ptrs(), bc=dm.bc_markers.ptr()),
+    }
+    ncells = dm.inner_cells()
+    out = {}
+    for name, fn in calls.items():
+        fn()                                               # warm-up
+        ms = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()                                           # returns with the stream drained
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        t = float(np.median(ms))
+        nbytes = LOW_MACH_BYTES[name] * ncells
+        gbs = nbytes / (t * 1e-3) / 1e9
+        out[name] = {"ms": round(t, 4), "algorithmic_bytes": nbytes, "achieved_GBs": round(gbs, 1),
+                     "frac_of_hbm_peak": round(gbs / PEAK_HBM_GBS, 4)}
+    return {"cells": ncells, "bound": "hbm", "peak_GBs": PEAK_HBM_GBS, "kernels": out,
+            "note": "median of %d calls, HIP events on the library's stream" % reps}
 
 
 def table_leg(args, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S):

@@ -309,6 +309,13 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt,
  * cells_inner, cells_integrated and kernel_ms.
  */
 int nrg_chem_set_table(nrg_chem_t *h, int32_t rows, const double *T_t, const double *dC_t);
+/* Host-array form of nrg_chem_table_step_dev for callers whose fields live in
+ * host memory (the reference's own data manager): staged like nrg_chem_step. */
+int nrg_chem_table_step(nrg_chem_t *h, const nrg_grid *grid, double dt,
+                        const double *T, const double *p, const double *rho,
+                        const double *const *Y, const int8_t *bc,
+                        double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                        int32_t *cell_status, nrg_chem_stats *stats);
 int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt,
                             const double *T, const double *p, const double *rho,
                             const double *const *Y, const int8_t *bc,

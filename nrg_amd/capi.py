@@ -325,7 +325,20 @@ def load_library(path=None):
     lib.nrg_chem_set_table.argtypes = [vp, C.c_int32, vp, vp]
     lib.nrg_chem_set_table.restype = C.c_int
     lib.nrg_chem_table_step_dev.argtypes = step_args
+    lib.nrg_chem_table_step.argtypes = step_args
     lib.nrg_chem_table_step_dev.restype = C.c_int
+    lib.nrg_chem_table_step.restype = C.c_int
+    # include/nrg_low_mach.h
+    lib.nrg_low_mach_state_equation_dev.argtypes = [vp, C.POINTER(NrgGrid), C.c_double, C.c_int32,
+                                                    vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp,
+                                                    vp, vp, vp, vp, vp, vp, vp]
+    lib.nrg_low_mach_state_equation_dev.restype = C.c_int
+    lib.nrg_low_mach_species_dev.argtypes = [vp, C.POINTER(NrgGrid), C.c_double, C.c_int32, vp, vp,
+                                             vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.nrg_low_mach_species_dev.restype = C.c_int
+    lib.nrg_low_mach_chem_divergence_dev.argtypes = [vp, C.POINTER(NrgGrid), vp, vp, vp, vp, vp,
+                                                     vp, vp, vp]
+    lib.nrg_low_mach_chem_divergence_dev.restype = C.c_int
     # include/nrg_mlp.h
     lib.nrg_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                    C.POINTER(vp)]

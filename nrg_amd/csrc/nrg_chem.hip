@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "../../include/nrg_chem.h"
+#include "../../include/nrg_low_mach.h"
 #include "../../include/nrg_surrogate.h"
 #include "chem_device.h"
 #include "mlp_internal.h"
@@ -653,6 +655,161 @@ __global__ void __launch_bounds__(TABLE_THREADS) table_kernel(TableArgs a)
   if (cnt) atomicAdd(&n_integ, cnt);
   __syncthreads();
   if (threadIdx.x == 0 && n_integ) atomicAdd(a.acc, n_integ);
+}
+
+// ---------------------------------------------------------------------------
+// Low-Mach (FDS) coupling (fds_low_mach_solver.f90, table_approximated_real_gas.
+// f90:570-677).  Per-cell, HBM-bound; the same aligned grid walk as the
+// table kernel, the mechanism staged in LDS once per block.
+// ---------------------------------------------------------------------------
+constexpr int LM_THREADS = 256;
+constexpr int LM_BLOCKS_PER_CU = 8;
+
+template <class F>
+NRG_DI void for_inner_cells(const GridIdx &g, F &&f)
+{
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long off = g.pad0 + (long long)blockIdx.x * blockDim.x + threadIdx.x; off < g.pad1;
+       off += stride)
+    if (inner_at(g, off)) f(off);
+}
+
+struct LowMachEosArgs {
+  GridIdx g;
+  double dt;
+  int predictor, ndim;
+  const MechCoef *mech;
+  const double *rho, *Y[S], *v[3], *dp;
+  const int8_t *bc;
+  double *p_stat, *p_stat_old, *T, *mmm;
+  double *p, *cp, *gamma, *h_s, *v_s, *e_i, *E_f;   // optional
+};
+
+__global__ void __launch_bounds__(LM_THREADS) low_mach_eos_kernel(LowMachEosArgs a)
+{
+  __shared__ MechCoef mc;
+  stage_mech(mc, a.mech);
+  for_inner_cells(a.g, [&](long long off) {
+    if (a.bc && a.bc[off] != 0) return;
+    double X[S], avg = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      X[s] = a.Y[s][off];
+      avg = avg + X[s] / mc.W[s];
+    }
+    const double W = 1.0 / avg;
+    a.mmm[off] = W;
+#pragma unroll
+    for (int s = 0; s < S; ++s) X[s] = (X[s] * W) / mc.W[s];
+    double ps;
+    if (a.predictor) {
+      const double p0 = a.p_stat[off];
+      a.p_stat_old[off] = p0;
+      ps = p0 + a.dp[off] * a.dt;
+    } else {
+      ps = 0.5 * ((a.p_stat[off] + a.p_stat_old[off]) + a.dp[off] * a.dt);
+    }
+    a.p_stat[off] = ps;
+    const double rho = a.rho[off];
+    const double T = ((ps / rho) / R_GAS_J) * W;
+    a.T[off] = T;
+    double cp = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) cp = cp + cp_molar(mc, T, s) * X[s];
+    const double g = cp / (cp - R_GAS_J);
+    if (a.gamma) a.gamma[off] = g;
+    if (a.h_s) {
+      double hT = 0.0, h0 = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) hT = hT + h_molar(mc, T, s) * X[s];
+#pragma unroll
+      for (int s = 0; s < S; ++s) h0 = h0 + mc.h_ref[s] * X[s];
+      a.h_s[off] = (hT - h0) / W;
+    }
+    if (a.cp) a.cp[off] = cp;
+    if (a.p) a.p[off] = ps;
+    if (a.v_s) a.v_s[off] = sqrt((g * ps) / rho);
+    const double ei = (ps / rho) / (g - 1.0);
+    if (a.e_i) a.e_i[off] = ei;
+    if (a.E_f && a.ndim > 0) {
+      const double vd = a.v[a.ndim - 1][off];
+      a.E_f[off] = ei + 0.5 * (vd * vd);
+    }
+  });
+}
+
+struct LowMachSpeciesArgs {
+  GridIdx g;
+  double dt;
+  int corrector;
+  const double *rho_a, *Y_a[S], *rho_b, *Y_b[S];
+  const double *rhs[S], *Yd[S], *Yc[S];   // rhs[0], Yd[0], Yc[0] NULL: term absent
+  const int8_t *bc;
+  double *rho_out, *Y_out[S];
+};
+
+__global__ void __launch_bounds__(LM_THREADS) low_mach_species_kernel(LowMachSpeciesArgs a)
+{
+  for_inner_cells(a.g, [&](long long off) {
+    if (a.bc && a.bc[off] != 0) return;
+    double r[S];
+    const double ra = a.rho_a[off];
+    const double rb = a.corrector ? a.rho_b[off] : 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double f = a.rhs[0] ? a.rhs[s][off] : 0.0;
+      if (!a.corrector) {
+        r[s] = ra * a.Y_a[s][off] + f * a.dt;
+        if (a.Yd[0]) r[s] = r[s] + a.Yd[s][off] * a.dt;
+        if (a.Yc[0]) r[s] = r[s] + a.Yc[s][off] * a.dt;
+      } else {
+        r[s] = 0.5 * (ra * a.Y_a[s][off] + rb * a.Y_b[s][off]) + f * (0.5 * a.dt);
+        if (a.Yd[0]) r[s] = r[s] + (0.5 * a.Yd[s][off]) * a.dt;
+        if (a.Yc[0]) r[s] = r[s] + (0.5 * a.Yc[s][off]) * a.dt;
+      }
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      r[s] = dmax(r[s], 0.0);
+      sum = sum + r[s];
+    }
+    if (sum > DBL_MIN) {
+      a.rho_out[off] = sum;
+#pragma unroll
+      for (int s = 0; s < S; ++s) a.Y_out[s][off] = r[s] / sum;
+    }
+  });
+}
+
+struct LowMachDivArgs {
+  GridIdx g;
+  const MechCoef *mech;
+  const double *rho, *T, *Y[S], *mmm, *Efc, *Yc[S];
+  const int8_t *bc;
+  double *div;
+};
+
+__global__ void __launch_bounds__(LM_THREADS) low_mach_div_kernel(LowMachDivArgs a)
+{
+  __shared__ MechCoef mc;
+  stage_mech(mc, a.mech);
+  for_inner_cells(a.g, [&](long long off) {
+    if (a.bc && a.bc[off] != 0) return;
+    const double W = a.mmm[off], rho = a.rho[off], T = a.T[off];
+    double d = 0.0;
+    d = d + a.Efc[off];
+    double cp = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) cp = cp + cp_molar(mc, T, s) * ((a.Y[s][off] * W) / mc.W[s]);
+    d = d / (((rho * cp) * T) / W);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double hs = (h_molar(mc, T, s) - mc.h_ref[s]) / mc.W[s];
+      d = d + (((W / mc.W[s]) / rho) - hs / (((rho * cp) * T) / W)) * a.Yc[s][off];
+    }
+    a.div[off] = d;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1670,12 +1827,13 @@ int nrg_chem_step_dev_diag(nrg_chem_t *h, const nrg_grid *grid, double dt, const
                        nstep, nfe, nje);
 }
 
-int nrg_chem_step(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
-                  const double *p, const double *rho, const double *const *Y, const int8_t *bc,
-                  double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
-                  int32_t *cell_status, nrg_chem_stats *stats)
+// Host fields: stage them through the handle's device buffers, run the FP64
+// step (or the table step) there, copy the outputs back.
+static int step_host(nrg_chem_t *h, bool table, const nrg_grid *grid, double dt, const double *T,
+                     const double *rho, const double *const *Y, const int8_t *bc,
+                     double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                     int32_t *cell_status, nrg_chem_stats *stats)
 {
-  (void)p;
   if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
   if (!grid || !T || !rho || !Y || !Y_prod || !Y_prod2 || !E_f_prod)
     return set_error(NRG_CHEM_E_ARG, "required pointer is NULL");
@@ -1717,8 +1875,10 @@ int nrg_chem_step(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *
   }
   int32_t *std_ = cell_status ? (int32_t *)h->stage[nd + 1] : nullptr;
   if (std_) HIP_TRY(hipMemcpyAsync(std_, cell_status, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  int rc = step_dev_impl(h, grid, dt, D(0), D(1), Yd, bcd, Ypd, Yp2d, D(2), std_, stats, nullptr,
-                         nullptr, nullptr);
+  int rc = table ? nrg_chem_table_step_dev(h, grid, dt, D(0), nullptr, D(1), Yd, bcd, Ypd, Yp2d, D(2),
+                                           std_, stats)
+                 : step_dev_impl(h, grid, dt, D(0), D(1), Yd, bcd, Ypd, Yp2d, D(2), std_, stats, nullptr,
+                                 nullptr, nullptr);
   if (rc != NRG_CHEM_OK && rc != NRG_CHEM_E_STEP_ZERO && rc != NRG_CHEM_E_REPEATED_FAIL) return rc;
   for (int q = 0; q < S; ++q) {
     HIP_TRY(hipMemcpyAsync(Y_prod[q], Ypd[q], bytes, hipMemcpyDeviceToHost, s));
@@ -1728,6 +1888,25 @@ int nrg_chem_step(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *
   if (std_) HIP_TRY(hipMemcpyAsync(cell_status, std_, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return rc;
+}
+
+int nrg_chem_step(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                  const double *p, const double *rho, const double *const *Y, const int8_t *bc,
+                  double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                  int32_t *cell_status, nrg_chem_stats *stats)
+{
+  (void)p;
+  return step_host(h, false, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status, stats);
+}
+
+int nrg_chem_table_step(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                        const double *p, const double *rho, const double *const *Y, const int8_t *bc,
+                        double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                        int32_t *cell_status, nrg_chem_stats *stats)
+{
+  (void)p;
+  if (h && !h->table) return set_error(NRG_CHEM_E_ARG, "no table (nrg_chem_set_table)");
+  return step_host(h, true, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status, stats);
 }
 
 static int closure_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, int apply,
@@ -1891,6 +2070,142 @@ int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, cons
   HIP_TRY(hipStreamSynchronize(h->stream));
   HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   if (stats) fill_stats(acc, a.g.ninner, ms, stats);
+  return NRG_CHEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Low-Mach coupling entry points (include/nrg_low_mach.h)
+// ---------------------------------------------------------------------------
+static long long lm_blocks(const nrg_chem_t *h, const GridIdx &g)
+{
+  return std::min<long long>((g.pad1 - g.pad0 + LM_THREADS - 1) / LM_THREADS,
+                             (long long)LM_BLOCKS_PER_CU * h->num_cu);
+}
+
+int nrg_low_mach_state_equation_dev(nrg_chem_t *h, const nrg_grid *grid, double time_step,
+                                    int32_t predictor, const double *rho,
+                                    const double *const *Y, const double *const *v, int32_t ndim,
+                                    const double *dp_stat_dt, const int8_t *bc, double *p_stat,
+                                    double *p_stat_old, double *T, double *mix_mol_mass, double *p,
+                                    double *mixture_cp, double *gamma, double *h_s, double *v_s,
+                                    double *e_i, double *E_f)
+{
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (!rho || !Y || !dp_stat_dt || !p_stat || !p_stat_old || !T || !mix_mol_mass)
+    return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL");
+  if (ndim < 0 || ndim > 3 || (ndim > 0 && !v)) return set_error(NRG_CHEM_E_ARG, "bad velocity");
+  if (!std::isfinite(time_step)) return set_error(NRG_CHEM_E_ARG, "time_step is not finite");
+  LowMachEosArgs a{};
+  int st = make_grid(grid, a.g);
+  if (st) return st;
+  for (int s = 0; s < S; ++s) {
+    if (!Y[s]) return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", s + 1);
+    a.Y[s] = Y[s];
+  }
+  for (int d = 0; d < ndim; ++d) {
+    if (!v[d]) return set_error(NRG_CHEM_E_ARG, "velocity %d is NULL", d + 1);
+    a.v[d] = v[d];
+  }
+  a.dt = time_step;
+  a.predictor = predictor != 0;
+  a.ndim = ndim;
+  a.mech = h->dev_mech;
+  a.rho = rho;
+  a.dp = dp_stat_dt;
+  a.bc = bc;
+  a.p_stat = p_stat;
+  a.p_stat_old = p_stat_old;
+  a.T = T;
+  a.mmm = mix_mol_mass;
+  a.p = p;
+  a.cp = mixture_cp;
+  a.gamma = gamma;
+  a.h_s = h_s;
+  a.v_s = v_s;
+  a.e_i = e_i;
+  a.E_f = E_f;
+  HIP_TRY(hipSetDevice(h->device));
+  if (a.g.ninner > 0) {
+    hipLaunchKernelGGL(low_mach_eos_kernel, dim3((unsigned)lm_blocks(h, a.g)), dim3(LM_THREADS), 0,
+                       h->stream, a);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return NRG_CHEM_OK;
+}
+
+int nrg_low_mach_species_dev(nrg_chem_t *h, const nrg_grid *grid, double time_step,
+                             int32_t corrector, const double *rho_a, const double *const *Y_a,
+                             const double *rho_b, const double *const *Y_b,
+                             const double *const *rhs, const double *const *Y_prod_diff,
+                             const double *const *Y_prod_chem, const int8_t *bc, double *rho_out,
+                             double *const *Y_out)
+{
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (!rho_a || !Y_a || !rho_out || !Y_out) return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL");
+  if (corrector && (!rho_b || !Y_b)) return set_error(NRG_CHEM_E_ARG, "the corrector needs rho_b and Y_b");
+  if (!std::isfinite(time_step)) return set_error(NRG_CHEM_E_ARG, "time_step is not finite");
+  LowMachSpeciesArgs a{};
+  int st = make_grid(grid, a.g);
+  if (st) return st;
+  for (int s = 0; s < S; ++s) {
+    if (!Y_a[s] || !Y_out[s] || (corrector && !Y_b[s]) || (rhs && !rhs[s]) ||
+        (Y_prod_diff && !Y_prod_diff[s]) || (Y_prod_chem && !Y_prod_chem[s]))
+      return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", s + 1);
+    a.Y_a[s] = Y_a[s];
+    a.Y_b[s] = corrector ? Y_b[s] : nullptr;
+    a.rhs[s] = rhs ? rhs[s] : nullptr;
+    a.Yd[s] = Y_prod_diff ? Y_prod_diff[s] : nullptr;
+    a.Yc[s] = Y_prod_chem ? Y_prod_chem[s] : nullptr;
+    a.Y_out[s] = Y_out[s];
+  }
+  a.dt = time_step;
+  a.corrector = corrector != 0;
+  a.rho_a = rho_a;
+  a.rho_b = corrector ? rho_b : nullptr;
+  a.bc = bc;
+  a.rho_out = rho_out;
+  HIP_TRY(hipSetDevice(h->device));
+  if (a.g.ninner > 0) {
+    hipLaunchKernelGGL(low_mach_species_kernel, dim3((unsigned)lm_blocks(h, a.g)), dim3(LM_THREADS), 0,
+                       h->stream, a);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return NRG_CHEM_OK;
+}
+
+int nrg_low_mach_chem_divergence_dev(nrg_chem_t *h, const nrg_grid *grid, const double *rho,
+                                     const double *T, const double *const *Y,
+                                     const double *mix_mol_mass, const double *E_f_prod_chem,
+                                     const double *const *Y_prod_chem, const int8_t *bc,
+                                     double *div_v)
+{
+  if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
+  if (!rho || !T || !Y || !mix_mol_mass || !E_f_prod_chem || !Y_prod_chem || !div_v)
+    return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL");
+  LowMachDivArgs a{};
+  int st = make_grid(grid, a.g);
+  if (st) return st;
+  for (int s = 0; s < S; ++s) {
+    if (!Y[s] || !Y_prod_chem[s]) return set_error(NRG_CHEM_E_ARG, "species %d field is NULL", s + 1);
+    a.Y[s] = Y[s];
+    a.Yc[s] = Y_prod_chem[s];
+  }
+  a.mech = h->dev_mech;
+  a.rho = rho;
+  a.T = T;
+  a.mmm = mix_mol_mass;
+  a.Efc = E_f_prod_chem;
+  a.bc = bc;
+  a.div = div_v;
+  HIP_TRY(hipSetDevice(h->device));
+  if (a.g.ninner > 0) {
+    hipLaunchKernelGGL(low_mach_div_kernel, dim3((unsigned)lm_blocks(h, a.g)), dim3(LM_THREADS), 0,
+                       h->stream, a);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return NRG_CHEM_OK;
 }
 

@@ -74,7 +74,7 @@ module nrg_chem_binding
   public :: nrg_chem_create, nrg_chem_destroy, nrg_chem_step, nrg_chem_step_dev
   public :: nrg_chem_closure_dev, nrg_state_equation_dev, nrg_energy_from_temperature_dev
   public :: nrg_chem_stream, nrg_chem_message
-  public :: nrg_chem_set_table, nrg_chem_table_step_dev
+  public :: nrg_chem_set_table, nrg_chem_table_step, nrg_chem_table_step_dev
 
   interface
     subroutine nrg_chem_default_opts(opts) bind(C, name="nrg_chem_default_opts")
@@ -146,7 +146,20 @@ module nrg_chem_binding
       integer(c_int) :: status
     end function
 
-    ! ODE_solver = 'table_approximated' (chemical_kinetics_solver.f90:250-346)
+    ! ODE_solver = 'table_approximated' (chemical_kinetics_solver.f90:250-346), host fields
+    function nrg_chem_table_step(handle, grid, dt, T, p, rho, Y, bc, Y_prod, Y_prod2, &
+                                 E_f_prod, cell_status, stats) &
+        bind(C, name="nrg_chem_table_step") result(status)
+      import :: c_int, c_ptr, c_double, nrg_grid, nrg_chem_stats
+      type(c_ptr), value :: handle
+      type(nrg_grid), intent(in) :: grid
+      real(c_double), value :: dt
+      type(c_ptr), value :: T, p, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status
+      type(nrg_chem_stats), intent(out) :: stats
+      integer(c_int) :: status
+    end function
+
+    ! the same on device-resident fields
     function nrg_chem_table_step_dev(handle, grid, dt, T, p, rho, Y, bc, Y_prod, Y_prod2, &
                                      E_f_prod, cell_status, stats) &
         bind(C, name="nrg_chem_table_step_dev") result(status)

@@ -9,8 +9,8 @@ prints, per kernel, the mean duration of its last N dispatches (the timed
 steps, after warm-up) from the per-dispatch trace.  ``counter``
 prints, per kernel, the average per-dispatch value of each collected counter
 (FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  With ``--calib-mib`` the
-largest dispatch of torch's direct device copy (the known-size copy bench.py
-runs with --copy-calibration-mib) is taken as the reference: its expected bytes divided
+largest single device-copy dispatch (the known-size copy bench.py runs with
+--copy-calibration-mib) is taken as the reference: its expected bytes divided
 by its measured bytes gives the factor applied to the other kernels.  Each
 calibrated byte counter also gets a line
     TRAFFIC <tab> kernel <tab> counter <tab> bytes per dispatch
@@ -57,22 +57,26 @@ def counters(path, calib_mib=None):
     name_key = "Kernel_Name" if "Kernel_Name" in rows[0] else "Kernel-Name"
     acc = defaultdict(lambda: defaultdict(float))
     disp = defaultdict(set)
+    per_disp = defaultdict(float)          # (kernel, counter, dispatch) -> value
     for r in rows:
         k = r[name_key]
         cname = r.get("Counter_Name") or r.get("Counter-Name")
-        acc[k][cname] += float(r.get("Counter_Value") or r.get("Counter-Value"))
-        disp[k].add(r.get("Dispatch_Id") or r.get("Dispatch-Id"))
+        did = r.get("Dispatch_Id") or r.get("Dispatch-Id")
+        val = float(r.get("Counter_Value") or r.get("Counter-Value"))
+        acc[k][cname] += val
+        disp[k].add(did)
+        per_disp[(k, cname, did)] += val
     factor = {}
     if calib_mib:
-        # torch's contiguous device copy (bench.py's known-size copy); concatenation
-        # and other copy-like kernels of the run are not calibration candidates
-        copies = [k for k in acc if "direct_copy" in k]
+        # bench.py's known-size device-to-device copy runs as the runtime's
+        # copyBuffer blit (torch's own direct-copy kernel where it copies
+        # itself); it is the largest single copy dispatch of the run.  Other
+        # copy-like kernels (concatenation) and per-name averages (which mix the
+        # calibration copy with smaller field clones) are not used.
         best = None
-        for k in copies:
-            for c, v in acc[k].items():
-                per = v / max(1, len(disp[k]))
-                if best is None or per > best[2]:
-                    best = (k, c, per)
+        for (k, c, _), v in per_disp.items():
+            if ("copyBuffer" in k or "direct_copy" in k) and (best is None or v > best[2]):
+                best = (k, c, v)
         if best:
             expected_kib = calib_mib * 1024.0
             for c in ("FETCH_SIZE", "WRITE_SIZE"):

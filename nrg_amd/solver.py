@@ -190,6 +190,36 @@ class ChemistryStep:
         check(self.lib, rc)
         return st
 
+    # -- low-Mach (FDS) coupling, include/nrg_low_mach.h -------------------------
+    def low_mach_state_equation_dev(self, grid, dt, predictor, rho, Y, dp_stat_dt, p_stat,
+                                    p_stat_old, T, mix_mol_mass, v=None, bc=None, p=None,
+                                    mixture_cp=None, gamma=None, h_s=None, v_s=None, e_i=None,
+                                    E_f=None):
+        _order_after_torch()
+        vv = capi.ptr_array(v) if v else None
+        rc = self.lib.nrg_low_mach_state_equation_dev(
+            self.h, C.byref(grid), float(dt), int(bool(predictor)), rho, capi.ptr_array(Y), vv,
+            len(v) if v else 0, dp_stat_dt, bc, p_stat, p_stat_old, T, mix_mol_mass, p, mixture_cp,
+            gamma, h_s, v_s, e_i, E_f)
+        check(self.lib, rc)
+
+    def low_mach_species_dev(self, grid, dt, corrector, rho_a, Y_a, rho_out, Y_out, rho_b=None,
+                             Y_b=None, rhs=None, Y_prod_diff=None, Y_prod_chem=None, bc=None):
+        _order_after_torch()
+        pa = lambda y: capi.ptr_array(y) if y else None  # noqa: E731
+        rc = self.lib.nrg_low_mach_species_dev(
+            self.h, C.byref(grid), float(dt), int(bool(corrector)), rho_a, pa(Y_a), rho_b, pa(Y_b),
+            pa(rhs), pa(Y_prod_diff), pa(Y_prod_chem), bc, rho_out, pa(Y_out))
+        check(self.lib, rc)
+
+    def low_mach_chem_divergence_dev(self, grid, rho, T, Y, mix_mol_mass, E_f_prod_chem,
+                                     Y_prod_chem, div_v, bc=None):
+        _order_after_torch()
+        rc = self.lib.nrg_low_mach_chem_divergence_dev(
+            self.h, C.byref(grid), rho, T, capi.ptr_array(Y), mix_mol_mass, E_f_prod_chem,
+            capi.ptr_array(Y_prod_chem), bc, div_v)
+        check(self.lib, rc)
+
     def closure_dev(self, grid, dt, rho, E_f_prod, Y_prod, E_f, Y, T, p=None, bc=None, v=None):
         _order_after_torch()
         vv = capi.ptr_array(v) if v else None

@@ -13,6 +13,7 @@
  */
 #include "nrg_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1080,6 +1081,114 @@ void or_table_cells(const nrg_chem_mech *m, int64_t ncells, double dt, double t_
     }
     E_f_prod[q] = ef;
     if (cell_status) cell_status[q] = NRG_CELL_OK;
+  }
+}
+
+/* ======================================================================
+ * Low-Mach (FDS) coupling: the per-cell consumers of the chemistry outputs in
+ * computing_module/src/current_build/fds_low_mach_solver.f90 and the low-Mach
+ * equation of state (table_approximated_real_gas.f90:570-677)
+ * ====================================================================== */
+
+void or_low_mach_eos_cells(const nrg_chem_mech *m, int64_t ncells, double dt, int predictor,
+                           const double *rho, const double *const *Y, const double *const *v,
+                           int ndim, const double *dp_stat_dt, const int8_t *bc, double *p_stat,
+                           double *p_stat_old, double *T, double *p, double *mix_mol_mass,
+                           double *mixture_cp, double *gamma, double *h_s, double *v_s,
+                           double *e_i, double *E_f)
+{
+  const int S = m->species_number;
+  for (int64_t q = 0; q < ncells; q++) {
+    if (bc && bc[q] != 0) continue;                                    /* :622 */
+    double avg = 0.0;                                                  /* :626-631 */
+    for (int s = 0; s < S; s++) avg = avg + Y[s][q] / m->molar_masses[s];
+    avg = 1.0 / avg;
+    const double W = avg;
+    mix_mol_mass[q] = W;
+    double X[OR_NMAX];
+    for (int s = 0; s < S; s++) X[s] = (Y[s][q] * W) / m->molar_masses[s];   /* :633-635 */
+    double ps;
+    if (predictor) {                                                   /* :637-642 */
+      p_stat_old[q] = p_stat[q];
+      ps = p_stat[q] + dp_stat_dt[q] * dt;
+    } else {
+      ps = 0.5 * ((p_stat[q] + p_stat_old[q]) + dp_stat_dt[q] * dt);
+    }
+    p_stat[q] = ps;
+    const double Tq = ((ps / rho[q]) / R_GAS_J) * W;                   /* :644 */
+    T[q] = Tq;
+    const double cp = or_mixture_cp_molar(m, Tq, X);                   /* :646-649 */
+    const double cv = cp - R_GAS_J;
+    const double g = cp / cv;
+    if (gamma) gamma[q] = g;
+    if (h_s) h_s[q] = (or_mixture_h_molar(m, Tq, X) - or_mixture_h_molar(m, T_REF, X)) / W;   /* :651 */
+    if (mixture_cp) mixture_cp[q] = cp;
+    if (p) p[q] = ps;                                                  /* :655 */
+    if (v_s) v_s[q] = sqrt((g * ps) / rho[q]);                         /* :658 */
+    const double ei = (ps / rho[q]) / (g - 1.0);                       /* :659 */
+    if (e_i) e_i[q] = ei;
+    if (E_f)
+      for (int d = 0; d < ndim; d++) E_f[q] = ei + 0.5 * (v[d][q] * v[d][q]);   /* :661-663 */
+  }
+}
+
+void or_low_mach_species_cells(const nrg_chem_mech *m, int64_t ncells, double dt, int corrector,
+                               const double *rho_a, const double *const *Y_a,
+                               const double *rho_b, const double *const *Y_b,
+                               const double *const *rhs, const double *const *Y_prod_diff,
+                               const double *const *Y_prod_chem, const int8_t *bc,
+                               double *rho_out, double *const *Y_out)
+{
+  const int S = m->species_number;
+  for (int64_t q = 0; q < ncells; q++) {
+    if (bc && bc[q] != 0) continue;                                    /* :848, :3044 */
+    double r[OR_NMAX];
+    for (int s = 0; s < S; s++) {
+      const double f = rhs ? rhs[s][q] : 0.0;                          /* rhs_vec = 0 (:865) + fluxes */
+      if (!corrector) {                                                /* :906-909 */
+        r[s] = rho_a[q] * Y_a[s][q] + f * dt;
+        if (Y_prod_diff) r[s] = r[s] + Y_prod_diff[s][q] * dt;
+        if (Y_prod_chem) r[s] = r[s] + Y_prod_chem[s][q] * dt;
+      } else {                                                         /* :3103-3107 */
+        r[s] = 0.5 * (rho_a[q] * Y_a[s][q] + rho_b[q] * Y_b[s][q]) + f * (0.5 * dt);
+        if (Y_prod_diff) r[s] = r[s] + (0.5 * Y_prod_diff[s][q]) * dt;
+        if (Y_prod_chem) r[s] = r[s] + (0.5 * Y_prod_chem[s][q]) * dt;
+      }
+    }
+    double sum = 0.0;                                                  /* :918-925, :3116-3123 */
+    for (int s = 0; s < S; s++) {
+      r[s] = dmax(r[s], 0.0);
+      sum = sum + r[s];
+    }
+    if (sum > DBL_MIN) {                                               /* tiny(1.0_dp) */
+      rho_out[q] = sum;
+      for (int s = 0; s < S; s++) Y_out[s][q] = r[s] / sum;
+    }
+  }
+}
+
+void or_low_mach_chem_divergence_cells(const nrg_chem_mech *m, int64_t ncells, const double *rho,
+                                       const double *T, const double *const *Y,
+                                       const double *mix_mol_mass, const double *E_f_prod_chem,
+                                       const double *const *Y_prod_chem, const int8_t *bc,
+                                       double *div_v)
+{
+  const int S = m->species_number;
+  for (int64_t q = 0; q < ncells; q++) {
+    if (bc && bc[q] != 0) continue;                                    /* :1034 */
+    const double W = mix_mol_mass[q];
+    double d = 0.0;                                                    /* :1057 */
+    d = d + E_f_prod_chem[q];                                          /* :1063 */
+    double X[OR_NMAX];
+    for (int s = 0; s < S; s++) X[s] = (Y[s][q] * W) / m->molar_masses[s];   /* :1075-1078 */
+    const double cp = or_mixture_cp_molar(m, T[q], X);                 /* :1117 */
+    d = d / (((rho[q] * cp) * T[q]) / W);                              /* :1119 */
+    for (int s = 0; s < S; s++) {
+      const double hs = (or_h_molar(m, T[q], s) - or_h_molar(m, T_REF, s)) / m->molar_masses[s];   /* :1123 */
+      d = d + (((W / m->molar_masses[s]) / rho[q]) - hs / (((rho[q] * cp) * T[q]) / W)) *
+                  Y_prod_chem[s][q];                                   /* :1161-1162 */
+    }
+    div_v[q] = d;
   }
 }
 

@@ -151,6 +151,41 @@ void or_table_cells(const nrg_chem_mech *m, int64_t ncells, double dt, double t_
                     double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
                     int32_t *cell_status);
 
+/* Low-Mach (FDS) coupling over a flat list of cells; cells with bc != 0 are
+ * skipped (left untouched), as in the reference loops.
+ *
+ * or_low_mach_eos_cells: apply_state_equation_low_mach_fds
+ *   (cm/table_approximated_real_gas.f90:570-677): W_mix, p_stat update
+ *   (predictor: p_stat_old = p_stat, p_stat += dp_stat_dt dt; corrector:
+ *   p_stat = (p_stat + p_stat_old + dp_stat_dt dt)/2), T = p_stat W_mix/(rho R),
+ *   cp, gamma, sensible h_s, p, sound speed, e_i and E_f = e_i + v_d^2/2 of
+ *   the LAST dimension d (the reference's loop overwrites E_f per dimension).
+ *   Optional outputs (NULL: not written): p, mixture_cp, gamma, h_s, v_s, e_i, E_f.
+ * or_low_mach_species_cells: the species update of calculate_interm_Y_predictor
+ *   (cm/fds_low_mach_solver.f90:906-925) / _corrector (:3103-3125) with the
+ *   flux divergence rhs supplied by the caller (NULL: zero) and the
+ *   diffusion and chemistry sources (NULL: flag off).
+ * or_low_mach_chem_divergence_cells: div_v of calculate_divergence_v
+ *   (cm/fds_low_mach_solver.f90:1057-1162) with the reactive terms only (heat
+ *   transfer, diffusion, radiation, particles off; zero face velocities). */
+void or_low_mach_eos_cells(const nrg_chem_mech *m, int64_t ncells, double dt, int predictor,
+                           const double *rho, const double *const *Y, const double *const *v,
+                           int ndim, const double *dp_stat_dt, const int8_t *bc, double *p_stat,
+                           double *p_stat_old, double *T, double *p, double *mix_mol_mass,
+                           double *mixture_cp, double *gamma, double *h_s, double *v_s,
+                           double *e_i, double *E_f);
+void or_low_mach_species_cells(const nrg_chem_mech *m, int64_t ncells, double dt, int corrector,
+                               const double *rho_a, const double *const *Y_a,
+                               const double *rho_b, const double *const *Y_b,
+                               const double *const *rhs, const double *const *Y_prod_diff,
+                               const double *const *Y_prod_chem, const int8_t *bc,
+                               double *rho_out, double *const *Y_out);
+void or_low_mach_chem_divergence_cells(const nrg_chem_mech *m, int64_t ncells, const double *rho,
+                                       const double *T, const double *const *Y,
+                                       const double *mix_mol_mass, const double *E_f_prod_chem,
+                                       const double *const *Y_prod_chem, const int8_t *bc,
+                                       double *div_v);
+
 void or_default_opts(nrg_chem_opts *o);
 
 #ifdef __cplusplus

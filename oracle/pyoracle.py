@@ -88,6 +88,13 @@ class Oracle:
         L.or_table_cells.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_double, C.c_int32, _dp,
                                      _dp, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _dp,
                                      C.c_void_p]
+        vp = C.c_void_p
+        L.or_low_mach_eos_cells.argtypes = [vp, C.c_int64, C.c_double, C.c_int, _dp, vp, vp, C.c_int,
+                                            _dp, vp, _dp, _dp, _dp, vp, _dp, vp, vp, vp, vp, vp, vp]
+        L.or_low_mach_species_cells.argtypes = [vp, C.c_int64, C.c_double, C.c_int, _dp, vp, vp, vp,
+                                                vp, vp, vp, vp, _dp, vp]
+        L.or_low_mach_chem_divergence_cells.argtypes = [vp, C.c_int64, _dp, _dp, vp, _dp, _dp, vp,
+                                                        vp, _dp]
         L.or_chem_ctx_size.restype = C.c_size_t
         L.or_chem_ctx_init.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         self.chem_fun = RHS_FN(C.cast(L.or_chem_fun, C.c_void_p).value)
@@ -243,6 +250,82 @@ class Oracle:
                                 _as_dp(T), _ptrs(list(Y)), bcp, _ptrs(list(yp)), _ptrs(list(yp2)),
                                 _as_dp(ef), st.ctypes.data)
         return dict(Y_prod=yp, Y_prod2=yp2, E_f_prod=ef, status=st)
+
+    # -- low-Mach (FDS) coupling -----------------------------------------------
+    def low_mach_eos(self, dt, predictor, rho, Y, p_stat, p_stat_old, dp_stat_dt, v=None,
+                     E_f=None, bc=None, fill=0.0):
+        """apply_state_equation_low_mach_fds over a flat list of cells.  Returns a
+        dict of the fields after the call (in/out fields start from the inputs,
+        pure outputs from `fill`)."""
+        n = len(rho)
+        f64 = lambda a: np.array(a, dtype=np.float64, order="C")  # noqa: E731
+        out = dict(p_stat=f64(p_stat), p_stat_old=f64(p_stat_old),
+                   E_f=f64(E_f) if E_f is not None else np.full(n, fill))
+        for k in ("T", "p", "mix_mol_mass", "mixture_cp", "gamma", "h_s", "v_s", "e_i"):
+            out[k] = np.full(n, fill)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        dp = np.ascontiguousarray(dp_stat_dt, dtype=np.float64)
+        ndim = 0
+        vptr = None
+        if v is not None:
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            ndim = v.shape[0]
+            vptr = _ptrs(list(v))
+        bcp = None
+        if bc is not None:
+            bc = np.ascontiguousarray(bc, dtype=np.int8)
+            bcp = bc.ctypes.data
+        o = out
+        self.lib.or_low_mach_eos_cells(self.mp, n, dt, int(bool(predictor)), _as_dp(rho),
+                                       _ptrs(list(Y)), vptr, ndim, _as_dp(dp), bcp,
+                                       _as_dp(o["p_stat"]), _as_dp(o["p_stat_old"]), _as_dp(o["T"]),
+                                       o["p"].ctypes.data, _as_dp(o["mix_mol_mass"]),
+                                       o["mixture_cp"].ctypes.data, o["gamma"].ctypes.data,
+                                       o["h_s"].ctypes.data, o["v_s"].ctypes.data,
+                                       o["e_i"].ctypes.data, o["E_f"].ctypes.data)
+        return out
+
+    def low_mach_species(self, dt, corrector, rho_a, Y_a, rho_b=None, Y_b=None, rhs=None,
+                         Y_prod_diff=None, Y_prod_chem=None, bc=None, rho_out=None, Y_out=None):
+        """Species update of calculate_interm_Y_predictor / _corrector.  Returns
+        (rho_out, Y_out); cells left untouched keep rho_out / Y_out's inputs
+        (zeros by default)."""
+        n = len(rho_a)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64) if a is not None else None  # noqa: E731
+        Y_a, Y_b, rhs, Y_prod_diff, Y_prod_chem = (c(a) for a in (Y_a, Y_b, rhs, Y_prod_diff,
+                                                                   Y_prod_chem))
+        pl = lambda a: _ptrs(list(a)) if a is not None else None  # noqa: E731
+        ro = np.array(rho_out, dtype=np.float64, order="C") if rho_out is not None else np.zeros(n)
+        yo = (np.array(Y_out, dtype=np.float64, order="C") if Y_out is not None
+              else np.zeros((self.S, n)))
+        bcp = None
+        if bc is not None:
+            bc = np.ascontiguousarray(bc, dtype=np.int8)
+            bcp = bc.ctypes.data
+        rb = c(rho_b) if rho_b is not None else None
+        self.lib.or_low_mach_species_cells(self.mp, n, dt, int(bool(corrector)), _as_dp(c(rho_a)),
+                                           pl(Y_a), rb.ctypes.data if rb is not None else None,
+                                           pl(Y_b), pl(rhs), pl(Y_prod_diff), pl(Y_prod_chem), bcp,
+                                           _as_dp(ro), _ptrs(list(yo)))
+        return ro, yo
+
+    def low_mach_chem_divergence(self, rho, T, Y, mix_mol_mass, E_f_prod_chem, Y_prod_chem,
+                                 bc=None, div_v=None):
+        n = len(rho)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        d = np.array(div_v, dtype=np.float64, order="C") if div_v is not None else np.zeros(n)
+        bcp = None
+        if bc is not None:
+            bc = np.ascontiguousarray(bc, dtype=np.int8)
+            bcp = bc.ctypes.data
+        Y = c(Y)
+        Yc = c(Y_prod_chem)
+        self.lib.or_low_mach_chem_divergence_cells(self.mp, n, _as_dp(c(rho)), _as_dp(c(T)),
+                                                   _ptrs(list(Y)), _as_dp(c(mix_mol_mass)),
+                                                   _as_dp(c(E_f_prod_chem)), _ptrs(list(Yc)), bcp,
+                                                   _as_dp(d))
+        return d
 
     def closure_cells(self, dt, rho, E_f_prod, Y_prod, E_f, Y, T, bc=None, nthreads=1):
         """Split closure + e->T.  Returns (E_f, Y, T, p) new arrays."""

@@ -95,6 +95,33 @@ def test_table_step_bitwise_vs_oracle(engine, oracle, mech, name):
     _report(name, integrated=integ, gated=T.size - integ, kernel_ms=g["stats"].kernel_ms)
 
 
+def test_host_array_table_step(engine, oracle, mech):
+    """nrg_chem_table_step on host arrays (staged through the handle's device
+    buffers) against the oracle, bit for bit."""
+    import ctypes as C
+    T_t, dC = TABLES["ascending"]
+    T, rho, Y, bc = _state(mech, n=3000, seed=6)
+    n = T.size
+    dt = 1e-7
+    engine.set_table(T_t, dC)
+    Yp = np.full((S, n), 3.0)
+    Yp2 = np.full((S, n), -7.0)
+    Ef = np.zeros(n)
+    status = np.full(n, -1, dtype=np.int32)
+    st = capi.NrgChemStats()
+    ptrs = lambda a: capi.ptr_array([a[s].ctypes.data for s in range(S)])  # noqa: E731
+    rc = engine.lib.nrg_chem_table_step(engine.h, C.byref(capi.flat_grid(n)), dt, T.ctypes.data,
+                                        None, rho.ctypes.data, ptrs(Y), bc.ctypes.data, ptrs(Yp),
+                                        ptrs(Yp2), Ef.ctypes.data, status.ctypes.data, C.byref(st))
+    capi.check(engine.lib, rc)
+    o = oracle.table_cells(dt, T_t, dC, T, Y, bc=bc, Y_prod2=np.full((S, n), -7.0))
+    assert np.array_equal(status, o["status"])
+    assert np.array_equal(_bits(Yp), _bits(o["Y_prod"]))
+    assert np.array_equal(_bits(Yp2), _bits(o["Y_prod2"]))
+    assert np.array_equal(_bits(Ef), _bits(o["E_f_prod"]))
+    assert st.cells_integrated == int((o["status"] == capi.NRG_CELL_OK).sum())
+
+
 def test_table_step_argument_errors(engine, mech):
     from nrg_amd.capi import NrgChemError
     from nrg_amd.solver import ChemistryStep
@@ -103,6 +130,12 @@ def test_table_step_argument_errors(engine, mech):
         T, rho, Y, bc = _state(mech, n=64)
         with pytest.raises(NrgChemError, match="no table"):
             _gpu_table_step(fresh, 1e-7, T, rho, Y, bc)
+        import ctypes as C
+        st = capi.NrgChemStats()
+        with pytest.raises(NrgChemError, match="no table"):
+            capi.check(fresh.lib, fresh.lib.nrg_chem_table_step(
+                fresh.h, C.byref(capi.flat_grid(T.size)), 1e-7, T.ctypes.data, None, rho.ctypes.data,
+                None, None, None, None, None, None, C.byref(st)))
         with pytest.raises(NrgChemError, match="2 rows"):
             fresh.set_table(np.array([300.0]), np.zeros((S, 1)))
         fresh.set_table(*TABLES["descending"])

@@ -16,9 +16,13 @@ def test_counter_summary_calibrates_and_emits_traffic(tmp_path):
     rows = [
         # a 512 MiB copy measured at 2.25x its bytes, a larger concatenation (not a
         # calibration candidate) and two chemistry dispatches
-        ("void at::native::elementwise_kernel_manual_unroll<128, 4, at::native::direct_copy_kernel_cuda>",
-         "FETCH_SIZE", 1, 512 * 1024 * 2.25),
+        ("__amd_rocclr_copyBuffer", "FETCH_SIZE", 1, 512 * 1024 * 2.25),
         ("void at::native::CatArrayBatchedCopy_contig<double>", "FETCH_SIZE", 4, 4096 * 1024.0),
+        # smaller dispatches of the same copy kernel (field clones) do not dilute it
+        ("void at::native::elementwise_kernel_manual_unroll<128, 4, at::native::direct_copy_kernel_cuda>",
+         "FETCH_SIZE", 5, 1024.0),
+        ("void at::native::elementwise_kernel_manual_unroll<128, 4, at::native::direct_copy_kernel_cuda>",
+         "FETCH_SIZE", 6, 2048.0),
         ("void (anonymous namespace)::chem_bdf_kernel<true, false>((anonymous namespace)::StepArgs)",
          "FETCH_SIZE", 2, 900.0),
         ("void (anonymous namespace)::chem_bdf_kernel<true, false>((anonymous namespace)::StepArgs)",
