@@ -75,6 +75,8 @@ module nrg_chem_binding
   public :: nrg_chem_closure_dev, nrg_state_equation_dev, nrg_energy_from_temperature_dev
   public :: nrg_chem_stream, nrg_chem_message
   public :: nrg_chem_set_table, nrg_chem_table_step, nrg_chem_table_step_dev
+  public :: nrg_low_mach_state_equation_dev, nrg_low_mach_species_dev
+  public :: nrg_low_mach_chem_divergence_dev
 
   interface
     subroutine nrg_chem_default_opts(opts) bind(C, name="nrg_chem_default_opts")
@@ -156,6 +158,48 @@ module nrg_chem_binding
       real(c_double), value :: dt
       type(c_ptr), value :: T, p, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status
       type(nrg_chem_stats), intent(out) :: stats
+      integer(c_int) :: status
+    end function
+
+    ! include/nrg_low_mach.h: apply_state_equation_low_mach_fds
+    ! (table_approximated_real_gas.f90:570-677) on device fields
+    function nrg_low_mach_state_equation_dev(handle, grid, dt, predictor, rho, Y, v, ndim, &
+                                             dp_stat_dt, bc, p_stat, p_stat_old, T, &
+                                             mix_mol_mass, p, mixture_cp, gamma, h_s, v_s, e_i, &
+                                             E_f) bind(C, name="nrg_low_mach_state_equation_dev") &
+        result(status)
+      import :: c_int, c_int32_t, c_ptr, c_double, nrg_grid
+      type(c_ptr), value :: handle
+      type(nrg_grid), intent(in) :: grid
+      real(c_double), value :: dt
+      integer(c_int32_t), value :: predictor, ndim
+      type(c_ptr), value :: rho, Y, v, dp_stat_dt, bc, p_stat, p_stat_old, T, mix_mol_mass
+      type(c_ptr), value :: p, mixture_cp, gamma, h_s, v_s, e_i, E_f
+      integer(c_int) :: status
+    end function
+
+    ! species predictor / corrector (fds_low_mach_solver.f90:906-925, 3103-3125)
+    function nrg_low_mach_species_dev(handle, grid, dt, corrector, rho_a, Y_a, rho_b, Y_b, rhs, &
+                                      Y_prod_diff, Y_prod_chem, bc, rho_out, Y_out) &
+        bind(C, name="nrg_low_mach_species_dev") result(status)
+      import :: c_int, c_int32_t, c_ptr, c_double, nrg_grid
+      type(c_ptr), value :: handle
+      type(nrg_grid), intent(in) :: grid
+      real(c_double), value :: dt
+      integer(c_int32_t), value :: corrector
+      type(c_ptr), value :: rho_a, Y_a, rho_b, Y_b, rhs, Y_prod_diff, Y_prod_chem, bc
+      type(c_ptr), value :: rho_out, Y_out
+      integer(c_int) :: status
+    end function
+
+    ! reactive part of div_v (fds_low_mach_solver.f90:1057-1162)
+    function nrg_low_mach_chem_divergence_dev(handle, grid, rho, T, Y, mix_mol_mass, &
+                                              E_f_prod_chem, Y_prod_chem, bc, div_v) &
+        bind(C, name="nrg_low_mach_chem_divergence_dev") result(status)
+      import :: c_int, c_ptr, nrg_grid
+      type(c_ptr), value :: handle
+      type(nrg_grid), intent(in) :: grid
+      type(c_ptr), value :: rho, T, Y, mix_mol_mass, E_f_prod_chem, Y_prod_chem, bc, div_v
       integer(c_int) :: status
     end function
 
