@@ -45,6 +45,7 @@ enum {
   NRG_CHEM_E_MECH = 2,         /* mechanism not supported by the compiled kernels             */
   NRG_CHEM_E_HIP = 3,          /* HIP runtime error (allocation, launch, copy)                */
   NRG_CHEM_E_NO_DEVICE = 4,    /* no gfx950 device available                                  */
+  NRG_CHEM_E_FORMAT = 5,       /* a mechanism / thermo / molar-mass file is unreadable        */
   NRG_CHEM_E_STEP_ZERO = 41,   /* a cell's step size went to zero  (SLATEC IERFLG 41)         */
   NRG_CHEM_E_REPEATED_FAIL = 42/* a cell failed MXTRY=50 attempts (SLATEC IERFLG 42)          */
 };

@@ -20,6 +20,7 @@ NRG_CHEM_E_ARG = 1
 NRG_CHEM_E_MECH = 2
 NRG_CHEM_E_HIP = 3
 NRG_CHEM_E_NO_DEVICE = 4
+NRG_CHEM_E_FORMAT = 5
 NRG_CHEM_E_STEP_ZERO = 41
 NRG_CHEM_E_REPEATED_FAIL = 42
 
@@ -328,6 +329,18 @@ def load_library(path=None):
     lib.nrg_chem_table_step.argtypes = step_args
     lib.nrg_chem_table_step_dev.restype = C.c_int
     lib.nrg_chem_table_step.restype = C.c_int
+    # include/nrg_mech.h
+    lib.nrg_mech_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_char_p,
+                                  C.POINTER(vp)]
+    lib.nrg_mech_load.restype = C.c_int
+    lib.nrg_mech_view.argtypes = [vp, C.POINTER(NrgChemMech)]
+    lib.nrg_mech_view.restype = C.c_int
+    lib.nrg_mech_species_name.argtypes = [vp, C.c_int32]
+    lib.nrg_mech_species_name.restype = C.c_char_p
+    lib.nrg_mech_reaction_string.argtypes = [vp, C.c_int32]
+    lib.nrg_mech_reaction_string.restype = C.c_char_p
+    lib.nrg_mech_free.argtypes = [vp]
+    lib.nrg_mech_free.restype = None
     # include/nrg_low_mach.h
     lib.nrg_low_mach_state_equation_dev.argtypes = [vp, C.POINTER(NrgGrid), C.c_double, C.c_int32,
                                                     vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp,

@@ -1452,6 +1452,7 @@ const char *nrg_chem_strerror(int st)
     case NRG_CHEM_E_MECH: return "mechanism not supported by the compiled kernels";
     case NRG_CHEM_E_HIP: return "HIP runtime error";
     case NRG_CHEM_E_NO_DEVICE: return "no gfx950 device";
+    case NRG_CHEM_E_FORMAT: return "unreadable mechanism, thermo or molar-mass file";
     case NRG_CHEM_E_STEP_ZERO: return "integrator step size went to zero in a cell (SLATEC IERFLG 41)";
     case NRG_CHEM_E_REPEATED_FAIL: return "integrator failed 50 attempts in a cell (SLATEC IERFLG 42)";
     default: return "unknown status";

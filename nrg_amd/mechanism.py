@@ -73,6 +73,8 @@ def _fortran_real(tok: str) -> float:
     if m:
         t = m.group(1) + "E" + m.group(2)
     t = t.replace("D", "E").replace("d", "e")
+    if not t or any(c not in "0123456789.eE+-" for c in t):
+        raise ValueError("not a real: %r" % tok)
     return float(t)
 
 
@@ -147,6 +149,24 @@ class _Records:
 def _read_lines(path):
     with open(path, "rb") as f:
         return f.read().decode("latin-1").splitlines()
+
+
+def _read_values(rec, nvals):
+    """``read(unit, *, iostat=error) word, v(1:nvals)``: the values convert in
+    order; an item that is not a real (a quoted efficiency record read as the
+    4th TROE value) ends the read with ``error`` set and leaves the remaining
+    values unassigned; the records read so far stay consumed."""
+    try:
+        items = rec.read_items(nvals + 1)
+    except EOFError:
+        return [], True
+    vals = []
+    for t in items[1:]:
+        try:
+            vals.append(_fortran_real(t))
+        except ValueError:
+            return vals, True
+    return vals, False
 
 
 def _species_names(lines):
@@ -291,8 +311,11 @@ def parse_mechanism(mech_path, default_enhanced_efficiencies=1.0, E_act_units="c
     rec = _Records(lines)
     # skip to REACTIONS
     while True:
-        if rec.read_items(1)[0] == "REACTIONS":
-            break
+        try:
+            if rec.read_items(1)[0] == "REACTIONS":
+                break
+        except EOFError:
+            raise MechanismError("no REACTIONS section") from None
 
     rtypes, coeffs, A, beta, E = [], [], [], [], []
     A_low, beta_low, E_low, troe, effs, rstrings = [], [], [], [], [], []
@@ -323,44 +346,58 @@ def parse_mechanism(mech_path, default_enhanced_efficiencies=1.0, E_act_units="c
         if third_body_position >= 0:
             enhanced_efficiency = True
         eff = np.full(S, float(default_enhanced_efficiencies))
-        tr = [0.0, 0.0, 0.0, 0.0]
-        a_lo = b_lo = e_lo = 0.0
         rec.backspace()
-        items = rec.read_items(4)
-        a, b, e = (_fortran_real(x) for x in items[1:4])
+        # every read below is iostat= (:391-418); unassigned values keep the
+        # zero the arrays were initialised with (:150-156)
+        vals, error = _read_values(rec, 3)
+        a, b, e = (vals + [0.0] * 3)[:3]
+        lo = [0.0, 0.0, 0.0]
+        tr = [0.0, 0.0, 0.0, 0.0]
+        eof = False                      # the file ends inside this reaction's records
         if fall_off >= 0:
-            tok = rec.read_items(1)[0]
-            if tok == "LOW":
-                rt = RT_IRREVERSIBLE_FALLOFF if r_type == 2 else RT_LINDEMANN
-                rec.backspace()
-                items = rec.read_items(4)
-                a_lo, b_lo, e_lo = (_fortran_real(x) for x in items[1:4])
-            tok = rec.read_items(1)[0]
-            if tok == "TROE":
-                rt = RT_IRREVERSIBLE_FALLOFF if r_type == 2 else RT_TROE
-                rec.backspace()
-                items = rec.read_items(5)
-                tr = [_fortran_real(x) for x in items[1:5]]
+            try:
+                tok = rec.read_items(1)[0]
+                error = False
+                if tok == "LOW":
+                    rt = RT_IRREVERSIBLE_FALLOFF if r_type == 2 else RT_LINDEMANN
+                    rec.backspace()
+                    vals, error = _read_values(rec, 3)
+                    lo = (vals + [0.0] * 3)[:3]
+                tok = rec.read_items(1)[0]
+                error = False
+                if tok == "TROE":
+                    rt = RT_IRREVERSIBLE_FALLOFF if r_type == 2 else RT_TROE
+                    rec.backspace()
+                    vals, error = _read_values(rec, 4)
+                    tr = (vals + [0.0] * 4)[:4]
+                else:
+                    rec.backspace()
+            except EOFError:
+                eof = True
+        a_lo, b_lo, e_lo = lo
+        # efficiency record (chemical_properties_class.f90:421-445), read only
+        # when the last read succeeded (:421)
+        if not error and not eof:
+            try:
+                tok = rec.read_items(1)[0]
+            except EOFError:
+                tok = ""
+            if "/" in tok and enhanced_efficiency:
+                rest = tok
+                while rest.strip():
+                    p1 = rest.find("/")
+                    if p1 < 0:
+                        break
+                    sname = rest[:p1].strip()
+                    rest = rest[p1 + 1:]
+                    p2 = rest.find("/")
+                    value = rest[:p2] if p2 >= 0 else rest
+                    sidx = index(sname)
+                    if 0 < sidx <= S:
+                        eff[sidx - 1] = _fortran_real(value)
+                    rest = rest[p2 + 1:] if p2 >= 0 else ""
             else:
                 rec.backspace()
-        # efficiency record (chemical_properties_class.f90:421-445)
-        try:
-            tok = rec.read_items(1)[0]
-        except EOFError:
-            tok = ""
-        if "/" in tok and enhanced_efficiency:
-            rest = tok
-            while rest.strip():
-                p1 = rest.find("/")
-                sname = rest[:p1].strip()
-                rest = rest[p1 + 1:]
-                p2 = rest.find("/")
-                sidx = index(sname)
-                if 0 < sidx <= S:
-                    eff[sidx - 1] = _fortran_real(rest[:p2])
-                rest = rest[p2 + 1:]
-        else:
-            rec.backspace()
         rtypes.append(rt)
         coeffs.append(([nl] + spl, [nr] + spr))
         A.append(a)
@@ -372,6 +409,8 @@ def parse_mechanism(mech_path, default_enhanced_efficiencies=1.0, E_act_units="c
         troe.append(tr)
         effs.append(eff)
         rstrings.append(original)
+        if eof:
+            break
 
     R = len(rtypes)
     cc = np.zeros((2, R, 4), dtype=np.int32)
@@ -517,6 +556,47 @@ def load_mechanism(mech_file="KEROMNES.txt", thermo_file="KEROMNES_THERMO.txt",
     a, mm = parse_thermo(d["species_names"], os.path.join(data_dir, thermo_file),
                          os.path.join(data_dir, molar_mass_file))
     return Mechanism(a_coeffs=a, molar_masses=mm, **d)
+
+
+def load_mechanism_native(mech_path, thermo_path, molar_mass_path,
+                          default_enhanced_efficiencies=1.0, E_act_units="cal.mol"):
+    """The same three files read by the library's C++ loader (include/nrg_mech.h,
+    nrg_amd/csrc/mech_loader.cpp); returns a Mechanism with copies of its arrays."""
+    import ctypes as C
+
+    from . import capi
+    lib = capi.load_library()
+    h = C.c_void_p()
+    capi.check(lib, lib.nrg_mech_load(str(mech_path).encode(), str(thermo_path).encode(),
+                                      str(molar_mass_path).encode(),
+                                      float(default_enhanced_efficiencies), E_act_units.encode(),
+                                      C.byref(h)))
+    try:
+        v = capi.NrgChemMech()
+        capi.check(lib, lib.nrg_mech_view(h, C.byref(v)))
+        S, R = v.species_number, v.reactions_number
+
+        def arr(ptr, shape, ct, dt):
+            n = int(np.prod(shape))
+            if n == 0:
+                return np.zeros(shape, dtype=dt)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(n,)).astype(dt).reshape(shape)
+
+        d = lambda ptr: arr(ptr, (R,), C.c_double, np.float64)  # noqa: E731
+        return Mechanism(
+            species_names=[lib.nrg_mech_species_name(h, s).decode("latin-1") for s in range(S)],
+            reactions_type=arr(v.reactions_type, (R,), C.c_int32, np.int32),
+            chemical_coeffs=arr(v.chemical_coeffs, (2, R, 4), C.c_int32, np.int32),
+            A=d(v.A), beta=d(v.beta), E_act=d(v.E_act), A_low=d(v.A_low),
+            beta_low=d(v.beta_low), E_act_low=d(v.E_act_low),
+            Troe_coeffs=arr(v.Troe_coeffs, (4, R), C.c_double, np.float64),
+            enhanced_efficiencies=arr(v.enhanced_efficiencies, (S, R), C.c_double, np.float64),
+            a_coeffs=arr(v.a_coeffs, (S, 2, 7), C.c_double, np.float64),
+            molar_masses=arr(v.molar_masses, (S,), C.c_double, np.float64),
+            reaction_strings=[lib.nrg_mech_reaction_string(h, r).decode("latin-1")
+                              for r in range(R)])
+    finally:
+        lib.nrg_mech_free(h)
 
 
 # Element composition of the KEROMNES species (for conservation checks and input synthesis).

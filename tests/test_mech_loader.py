@@ -1,0 +1,185 @@
+"""The library's C++ mechanism loader (include/nrg_mech.h) against the Python
+loader and against the reference's reading rules
+(package_library/src/chemical_properties_class.f90:289-451,
+thermophysical_properties_class.f90:137-179, 1226-1341): the same arrays, bit
+for bit, for KEROMNES and the other mechanisms NRG ships (tests/golden/
+mechanisms), and the rules worked out by hand on small files."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from nrg_amd import capi
+from nrg_amd import mechanism as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden", "mechanisms")
+DATA = M.DATA_DIR
+MM = os.path.join(DATA, "molar_masses.dat")
+K_CAL = M.R_GAS_J / M.R_GAS_CAL
+
+CASES = {
+    "KEROMNES": (os.path.join(DATA, "KEROMNES.txt"), os.path.join(DATA, "KEROMNES_THERMO.txt")),
+    "KONNOV": (os.path.join(GOLD, "KONNOV.txt"), os.path.join(GOLD, "KONNOV_THERMO.txt")),
+    "ZHANG": (os.path.join(GOLD, "ZHANG.txt"), os.path.join(GOLD, "ZHANG_THERMO.txt")),
+    "TEREZA": (os.path.join(GOLD, "TEREZA.txt"), os.path.join(GOLD, "TEREZA_THERMO.txt")),
+    "ACETYLENE": (os.path.join(GOLD, "ACETYLENE_Varatharajan.txt"),
+                  os.path.join(GOLD, "FFCM-1_thermo.txt")),
+}
+ARRAYS = ("reactions_type", "chemical_coeffs", "A", "beta", "E_act", "A_low", "beta_low",
+          "E_act_low", "Troe_coeffs", "enhanced_efficiencies", "a_coeffs", "molar_masses")
+
+
+def _python(mech, thermo, **kw):
+    d = M.parse_mechanism(mech, **kw)
+    a, mm = M.parse_thermo(d["species_names"], thermo, MM)
+    return M.Mechanism(a_coeffs=a, molar_masses=mm, **d)
+
+
+def _same(p, n):
+    assert p.species_names == n.species_names
+    assert p.reaction_strings == n.reaction_strings
+    for k in ARRAYS:
+        a, b = getattr(p, k), getattr(n, k)
+        assert a.shape == b.shape and a.dtype == b.dtype, k
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), k
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_native_loader_matches_python_bitwise(name):
+    mech, thermo = CASES[name]
+    _same(_python(mech, thermo), M.load_mechanism_native(mech, thermo, MM))
+
+
+@pytest.mark.parametrize("units", ["J.mol", "kJ.mol", "kcal.mol"])
+def test_energy_units_and_default_efficiency(units):
+    mech, thermo = CASES["KEROMNES"]
+    p = _python(mech, thermo, default_enhanced_efficiencies=0.5, E_act_units=units)
+    n = M.load_mechanism_native(mech, thermo, MM, 0.5, units)
+    _same(p, n)
+    ref = M.load_mechanism()
+    coef = {"J.mol": 1.0, "kJ.mol": 1000.0, "kcal.mol": K_CAL * 1000.0}[units]
+    assert np.allclose(n.E_act, ref.E_act / K_CAL * coef, rtol=1e-15, atol=0)
+    assert np.all(n.enhanced_efficiencies[:, ref.reactions_type == M.RT_ELEMENTARY] == 0.5)
+
+
+def test_keromnes_native_is_what_the_step_uses():
+    mech, thermo = CASES["KEROMNES"]
+    _same(M.load_mechanism(), M.load_mechanism_native(mech, thermo, MM))
+
+
+def test_konnov_reading_rules_by_hand():
+    """KONNOV.txt exercises three rules: unquoted efficiency lines end at the
+    first '/' and are not efficiencies; A scales with the left-hand count;
+    'TROE 0.5 1 1E+10' has three values, so the quoted efficiency record is
+    read as the fourth, the read fails (iostat), Troe(4) stays 0 and the
+    reaction keeps the default efficiencies."""
+    mech, thermo = CASES["KONNOV"]
+    m = M.load_mechanism_native(mech, thermo, MM)
+    r0 = m.reaction_strings.index("H+H+M=H2+M")
+    assert m.reactions_type[r0] == M.RT_THIRD_BODY
+    assert m.A[r0] == 7.000e17 * 1.0e-12                     # H + H + M: three terms
+    assert np.all(m.enhanced_efficiencies[:, r0] == 1.0)      # " H2/0.0/ ..." unquoted
+    r1 = m.reaction_strings.index("H2O+M=H+OH+M")
+    assert m.A[r1] == 6.060e27 * 1.0e-06 and m.E_act[r1] == 120770.0 * K_CAL
+    rt = m.reaction_strings.index("H+O2(+M)=HO2(+M)")
+    assert m.reactions_type[rt] == M.RT_TROE
+    assert m.A[rt] == 4.660e12 * 1.0e-06 and m.A_low[rt] == 1.225e19 * 1.0e-12
+    assert list(m.Troe_coeffs[:, rt]) == [0.5, 1.0, 1.0e10, 0.0]
+    assert np.all(m.enhanced_efficiencies[:, rt] == 1.0)
+    assert "OX" in m.species_names and m.species_number == 13
+
+
+THERMO = os.path.join(DATA, "KEROMNES_THERMO.txt")
+
+
+def _write(tmp_path, body):
+    p = tmp_path / "mech.txt"
+    p.write_text("ELEMENTS\nH O\nEND\nSPECIES\nH2 O2 H O OH HO2\nEND\nREACTIONS\n" + body)
+    return str(p)
+
+
+def test_sticky_efficiency_flag_unknown_species_and_exponents(tmp_path):
+    """The efficiency flag is off until the first '+M' reaction and then stays
+    on (an elementary reaction after it reads its quoted record); a reaction
+    with an unknown species is dropped; '4.8749+04' reads as 48749."""
+    mech = _write(tmp_path, "\n".join([
+        "H+O2=O+OH   1.0E+10  0.0  4.8749+04",
+        "'H2/2.5/'",                          # flag still off: not read, re-read as a non-reaction
+        "H+XX=HO2    1.0      0.0  0.0",      # unknown species: dropped
+        "H+H+M=H2+M  2.0E+12  -1.0 0.0",
+        "'H2/0.0/ O2/1.5/'",
+        "O+O=O2      3.0E+12  0.0  0.0",
+        "'H2/3.0/'",                          # flag sticky: read for an elementary reaction
+        "END", ""]))
+    p = _python(mech, THERMO)
+    n = M.load_mechanism_native(mech, THERMO, MM)
+    _same(p, n)
+    assert n.reaction_strings == ["H+O2=O+OH", "H+H+M=H2+M", "O+O=O2"]
+    assert n.E_act[0] == 48749.0 * K_CAL
+    h2, o2 = n.species_names.index("H2"), n.species_names.index("O2")
+    assert n.enhanced_efficiencies[h2, 0] == 1.0
+    assert n.enhanced_efficiencies[h2, 1] == 0.0 and n.enhanced_efficiencies[o2, 1] == 1.5
+    assert n.enhanced_efficiencies[h2, 2] == 3.0
+    assert list(n.reactions_type) == [M.RT_ELEMENTARY, M.RT_THIRD_BODY, M.RT_ELEMENTARY]
+
+
+def test_irreversible_and_falloff_types_and_end_of_file(tmp_path):
+    mech = _write(tmp_path, "\n".join([
+        "H+O2=>O+OH       1.0 0.0 0.0",
+        "H+H+M=>H2+M      1.0 0.0 0.0",
+        "H+O2(+M)=>HO2(+M) 1.0 0.0 0.0",
+        "   LOW 2.0 0.0 0.0",
+        "   TROE 0.5 10.0 100.0 1000.0",
+        "O+H(+M)<=>OH(+M)  1.0 0.0 0.0",      # the file ends inside its records
+        ]))
+    p = _python(mech, THERMO)
+    n = M.load_mechanism_native(mech, THERMO, MM)
+    _same(p, n)
+    assert list(n.reactions_type) == [M.RT_IRREVERSIBLE, M.RT_IRREVERSIBLE_THIRD_BODY,
+                                      M.RT_IRREVERSIBLE_FALLOFF, M.RT_THIRD_BODY]
+    assert list(n.Troe_coeffs[:, 2]) == [0.5, 10.0, 100.0, 1000.0]
+    assert n.A_low[2] == 2.0 * 1.0e-12                       # two left terms + 1
+
+
+def test_format_errors(tmp_path):
+    lib = capi.load_library()
+    h = C.c_void_p()
+    bad = tmp_path / "nomm.dat"
+    bad.write_text("H2 2.016E-3\n")
+    mech, thermo = CASES["KEROMNES"]
+    for args in ((mech, thermo, str(bad)), (str(tmp_path / "missing.txt"), thermo, MM)):
+        st = lib.nrg_mech_load(*(a.encode() for a in args), 1.0, b"cal.mol", C.byref(h))
+        assert st == capi.NRG_CHEM_E_FORMAT
+        assert lib.nrg_chem_last_error()
+    st = lib.nrg_mech_load(mech.encode(), thermo.encode(), MM.encode(), 1.0, b"eV", C.byref(h))
+    assert st == capi.NRG_CHEM_E_FORMAT and b"E_act_units" in lib.nrg_chem_last_error()
+    with pytest.raises(M.MechanismError):
+        M.parse_thermo(["H2", "O2"], thermo, str(bad))
+
+
+def test_native_view_feeds_create():
+    """nrg_chem_create takes the loader's view: KEROMNES passes the topology
+    check (then needs a device), KONNOV is refused as a mechanism the compiled
+    kernels do not cover."""
+    import torch
+    lib = capi.load_library()
+    opts = capi.default_opts()
+    for name, want in (("KEROMNES", None), ("KONNOV", capi.NRG_CHEM_E_MECH)):
+        mech, thermo = CASES[name]
+        h = C.c_void_p()
+        capi.check(lib, lib.nrg_mech_load(mech.encode(), thermo.encode(), MM.encode(), 1.0,
+                                          b"cal.mol", C.byref(h)))
+        v = capi.NrgChemMech()
+        capi.check(lib, lib.nrg_mech_view(h, C.byref(v)))
+        out = C.c_void_p()
+        st = lib.nrg_chem_create(C.byref(v), C.byref(opts), C.byref(out))
+        if want is None:
+            assert st in (capi.NRG_CHEM_OK, capi.NRG_CHEM_E_NO_DEVICE)
+            if st == capi.NRG_CHEM_OK:
+                assert torch.cuda.is_available()
+                lib.nrg_chem_destroy(out)
+        else:
+            assert st == want
+        lib.nrg_mech_free(h)
