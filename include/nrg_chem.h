@@ -297,6 +297,7 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt,
  * 250-346, interpolate_chemical_kinetics :650-685).
  *
  * nrg_chem_set_table replaces read_chemical_kinetics_table (:608-648): `rows`
+ * (2 .. 2^24)
  * temperatures T_t and, species-major, dC_t[s*rows + i] (the table's columns
  * 2..S+1, i.e. Y_prod2 of the run that wrote it, write_chemical_kinetics_table
  * :542-606).  The table is copied to the device.

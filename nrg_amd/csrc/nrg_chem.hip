@@ -2001,6 +2001,7 @@ int nrg_chem_set_table(nrg_chem_t *h, int32_t rows, const double *T_t, const dou
 {
   if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
   if (rows < 2 || !T_t || !dC_t) return set_error(NRG_CHEM_E_ARG, "a table needs >= 2 rows and both arrays");
+  if (rows > (1 << 24)) return set_error(NRG_CHEM_E_ARG, "a table has at most 2^24 rows, got %d", rows);
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->table) HIP_TRY(hipFree(h->table));
