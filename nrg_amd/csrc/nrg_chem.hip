@@ -63,6 +63,20 @@ NRG_DI bool inner_at(const GridIdx &g, long long p)
          (unsigned long long)(k - (g.ilo2 - g.lo2)) < (unsigned long long)g.iz;
 }
 
+// The aligned walk: thread t of the launch takes allocation offset pad0 + t,
+// from the 64-cell slice of the first inner cell to the last inner cell, and
+// skips ghost cells.  With ghost layers a row of inner cells starts off a cache
+// line; walking inner cells in order would make every wave touch one more line
+// per field.  One offset per thread: a grid-stride loop here let the compiler
+// hoist the LDS-staged mechanism into registers (closure 72 -> 169 VGPRs) and
+// was slower for every kernel that uses it.
+template <class F>
+NRG_DI void at_inner_cell(const GridIdx &g, F &&f)
+{
+  const long long off = g.pad0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (off < g.pad1 && inner_at(g, off)) f(off);
+}
+
 NRG_DI long long cell_offset(const GridIdx &g, long long q)
 {
   const long long i = q % g.ix + g.ilo0;
@@ -450,48 +464,47 @@ __global__ void __launch_bounds__(256) closure_kernel(ClosureArgs a)
 {
   __shared__ MechCoef mc;
   stage_mech(mc, a.mech);
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= a.g.ninner) return;
-  const long long off = cell_offset(a.g, q);
-  if (a.bc && a.bc[off] != 0) return;
-  const double rho = a.rho[off];
-  double Ef = a.E_f[off];
-  double y[S];
-  if (a.apply_sources) {
-    const double efp = 0.0 + a.Efp[off] * a.dt;
-    Ef = Ef + efp / rho;
-    double sum = 0.0;
+  at_inner_cell(a.g, [&](long long off) {
+    if (a.bc && a.bc[off] != 0) return;
+    const double rho = a.rho[off];
+    double Ef = a.E_f[off];
+    double y[S];
+    if (a.apply_sources) {
+      const double efp = 0.0 + a.Efp[off] * a.dt;
+      Ef = Ef + efp / rho;
+      double sum = 0.0;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const double ypr = 0.0 + a.Yp[s][off] * a.dt;
-      y[s] = a.Y[s][off] + ypr / rho;
-      sum = sum + y[s];
+      for (int s = 0; s < S; ++s) {
+        const double ypr = 0.0 + a.Yp[s][off] * a.dt;
+        y[s] = a.Y[s][off] + ypr / rho;
+        sum = sum + y[s];
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s) y[s] = dmax(y[s], 0.0) / sum;
+      a.E_f[off] = Ef;
+#pragma unroll
+      for (int s = 0; s < S; ++s) a.Y[s][off] = y[s];
+    } else {
+#pragma unroll
+      for (int s = 0; s < S; ++s) y[s] = a.Y[s][off];
     }
+    double e_i = Ef;
+    for (int d = 0; d < a.ndim; ++d) {
+      const double vd = a.v[d][off];
+      e_i = e_i - 0.5 * (vd * vd);
+    }
+    double avg = 0.0;
 #pragma unroll
-    for (int s = 0; s < S; ++s) y[s] = dmax(y[s], 0.0) / sum;
-    a.E_f[off] = Ef;
+    for (int s = 0; s < S; ++s) avg = avg + y[s] / mc.W[s];
+    const double wmix = 1.0 / avg;
+    double X[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) a.Y[s][off] = y[s];
-  } else {
-#pragma unroll
-    for (int s = 0; s < S; ++s) y[s] = a.Y[s][off];
-  }
-  double e_i = Ef;
-  for (int d = 0; d < a.ndim; ++d) {
-    const double vd = a.v[d][off];
-    e_i = e_i - 0.5 * (vd * vd);
-  }
-  double avg = 0.0;
-#pragma unroll
-  for (int s = 0; s < S; ++s) avg = avg + y[s] / mc.W[s];
-  const double wmix = 1.0 / avg;
-  double X[S];
-#pragma unroll
-  for (int s = 0; s < S; ++s) X[s] = (y[s] / mc.W[s]) * wmix;
-  e_i = e_i * wmix;
-  const double Tn = calculate_temperature(mc, a.T[off], e_i, X);
-  a.T[off] = Tn;
-  if (a.p) a.p[off] = ((Tn * rho) * R_GAS_J) / wmix;
+    for (int s = 0; s < S; ++s) X[s] = (y[s] / mc.W[s]) * wmix;
+    e_i = e_i * wmix;
+    const double Tn = calculate_temperature(mc, a.T[off], e_i, X);
+    a.T[off] = Tn;
+    if (a.p) a.p[off] = ((Tn * rho) * R_GAS_J) / wmix;
+  });
 }
 
 struct EnergyArgs {
@@ -582,10 +595,9 @@ __device__ __forceinline__ int table_interval(const double *Tt, int rows, int as
 }
 
 constexpr int TABLE_THREADS = 256;
-constexpr int TABLE_BLOCKS_PER_CU = 4;        // four 4-wave blocks: one wave per SIMD each
 constexpr int TABLE_LDS_BYTES = 32768;        // a table up to this size is staged in LDS
 
-// A grid of TABLE_BLOCKS_PER_CU blocks per CU strides over the cells, so the
+// A grid of as many blocks as fit on the CUs strides over the cells, so the
 // table is staged into LDS once per block (not once per 256 cells) and the
 // integrated-cell count takes one global atomic per block.
 template <bool TAB_LDS>
@@ -659,20 +671,10 @@ __global__ void __launch_bounds__(TABLE_THREADS) table_kernel(TableArgs a)
 
 // ---------------------------------------------------------------------------
 // Low-Mach (FDS) coupling (fds_low_mach_solver.f90, table_approximated_real_gas.
-// f90:570-677).  Per-cell, HBM-bound; the same aligned grid walk as the
-// table kernel, the mechanism staged in LDS once per block.
+// f90:570-677).  Per-cell, HBM-bound; one thread per offset of the aligned
+// walk, the mechanism staged in LDS per block.
 // ---------------------------------------------------------------------------
 constexpr int LM_THREADS = 256;
-constexpr int LM_BLOCKS_PER_CU = 8;
-
-template <class F>
-NRG_DI void for_inner_cells(const GridIdx &g, F &&f)
-{
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long off = g.pad0 + (long long)blockIdx.x * blockDim.x + threadIdx.x; off < g.pad1;
-       off += stride)
-    if (inner_at(g, off)) f(off);
-}
 
 struct LowMachEosArgs {
   GridIdx g;
@@ -689,7 +691,7 @@ __global__ void __launch_bounds__(LM_THREADS) low_mach_eos_kernel(LowMachEosArgs
 {
   __shared__ MechCoef mc;
   stage_mech(mc, a.mech);
-  for_inner_cells(a.g, [&](long long off) {
+  at_inner_cell(a.g, [&](long long off) {
     if (a.bc && a.bc[off] != 0) return;
     double X[S], avg = 0.0;
 #pragma unroll
@@ -750,7 +752,7 @@ struct LowMachSpeciesArgs {
 
 __global__ void __launch_bounds__(LM_THREADS) low_mach_species_kernel(LowMachSpeciesArgs a)
 {
-  for_inner_cells(a.g, [&](long long off) {
+  at_inner_cell(a.g, [&](long long off) {
     if (a.bc && a.bc[off] != 0) return;
     double r[S];
     const double ra = a.rho_a[off];
@@ -794,7 +796,7 @@ __global__ void __launch_bounds__(LM_THREADS) low_mach_div_kernel(LowMachDivArgs
 {
   __shared__ MechCoef mc;
   stage_mech(mc, a.mech);
-  for_inner_cells(a.g, [&](long long off) {
+  at_inner_cell(a.g, [&](long long off) {
     if (a.bc && a.bc[off] != 0) return;
     const double W = a.mmm[off], rho = a.rho[off], T = a.T[off];
     double d = 0.0;
@@ -1423,6 +1425,22 @@ int check_device(int dev)
 
 }  // namespace
 
+// Blocks of 256 threads covering the aligned walk, one offset per thread.
+static long long walk_blocks(const GridIdx &g) { return std::max<long long>(1, (g.pad1 - g.pad0 + 255) / 256); }
+
+// Blocks for a kernel that strides over the aligned walk itself: its resident
+// blocks per CU times the CUs (a block more than fits would run as a tail
+// after the others), and no more blocks than the walk has 256-cell slices.
+template <class K>
+static long long resident_blocks(const nrg_chem_handle *h, K kernel, size_t dyn_lds, const GridIdx &g)
+{
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, dyn_lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return std::min<long long>(walk_blocks(g), (long long)per_cu * h->num_cu);
+}
+
 extern "C" {
 
 int nrg_chem_abi_version(void) { return NRG_CHEM_ABI_VERSION; }
@@ -1944,7 +1962,7 @@ static int closure_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, int appl
   a.p = p;
   HIP_TRY(hipSetDevice(h->device));
   if (a.g.ninner > 0) {
-    const long long nb = (a.g.ninner + 255) / 256;
+    const long long nb = walk_blocks(a.g);
     hipLaunchKernelGGL(closure_kernel, dim3((unsigned)nb), dim3(256), 0, h->stream, a);
     HIP_TRY(hipGetLastError());
   }
@@ -2057,14 +2075,13 @@ int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, cons
   float ms = 0.f;
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   if (a.g.ninner > 0) {
-    const long long nb = std::min<long long>((a.g.pad1 - a.g.pad0 + TABLE_THREADS - 1) / TABLE_THREADS,
-                                             (long long)TABLE_BLOCKS_PER_CU * h->num_cu);
     const size_t tab_bytes = sizeof(double) * (size_t)(S + 1) * (size_t)a.rows;
     if (tab_bytes <= (size_t)TABLE_LDS_BYTES)
-      hipLaunchKernelGGL(table_kernel<true>, dim3((unsigned)nb), dim3(TABLE_THREADS), tab_bytes,
-                         h->stream, a);
+      hipLaunchKernelGGL(table_kernel<true>, dim3((unsigned)resident_blocks(h, table_kernel<true>, tab_bytes, a.g)),
+                         dim3(TABLE_THREADS), tab_bytes, h->stream, a);
     else
-      hipLaunchKernelGGL(table_kernel<false>, dim3((unsigned)nb), dim3(TABLE_THREADS), 0, h->stream, a);
+      hipLaunchKernelGGL(table_kernel<false>, dim3((unsigned)resident_blocks(h, table_kernel<false>, 0, a.g)),
+                         dim3(TABLE_THREADS), 0, h->stream, a);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
@@ -2078,12 +2095,6 @@ int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, cons
 // ---------------------------------------------------------------------------
 // Low-Mach coupling entry points (include/nrg_low_mach.h)
 // ---------------------------------------------------------------------------
-static long long lm_blocks(const nrg_chem_t *h, const GridIdx &g)
-{
-  return std::min<long long>((g.pad1 - g.pad0 + LM_THREADS - 1) / LM_THREADS,
-                             (long long)LM_BLOCKS_PER_CU * h->num_cu);
-}
-
 int nrg_low_mach_state_equation_dev(nrg_chem_t *h, const nrg_grid *grid, double time_step,
                                     int32_t predictor, const double *rho,
                                     const double *const *Y, const double *const *v, int32_t ndim,
@@ -2128,7 +2139,7 @@ int nrg_low_mach_state_equation_dev(nrg_chem_t *h, const nrg_grid *grid, double 
   a.E_f = E_f;
   HIP_TRY(hipSetDevice(h->device));
   if (a.g.ninner > 0) {
-    hipLaunchKernelGGL(low_mach_eos_kernel, dim3((unsigned)lm_blocks(h, a.g)), dim3(LM_THREADS), 0,
+    hipLaunchKernelGGL(low_mach_eos_kernel, dim3((unsigned)walk_blocks(a.g)), dim3(LM_THREADS), 0,
                        h->stream, a);
     HIP_TRY(hipGetLastError());
   }
@@ -2169,7 +2180,7 @@ int nrg_low_mach_species_dev(nrg_chem_t *h, const nrg_grid *grid, double time_st
   a.rho_out = rho_out;
   HIP_TRY(hipSetDevice(h->device));
   if (a.g.ninner > 0) {
-    hipLaunchKernelGGL(low_mach_species_kernel, dim3((unsigned)lm_blocks(h, a.g)), dim3(LM_THREADS), 0,
+    hipLaunchKernelGGL(low_mach_species_kernel, dim3((unsigned)walk_blocks(a.g)), dim3(LM_THREADS), 0,
                        h->stream, a);
     HIP_TRY(hipGetLastError());
   }
@@ -2203,7 +2214,7 @@ int nrg_low_mach_chem_divergence_dev(nrg_chem_t *h, const nrg_grid *grid, const 
   a.div = div_v;
   HIP_TRY(hipSetDevice(h->device));
   if (a.g.ninner > 0) {
-    hipLaunchKernelGGL(low_mach_div_kernel, dim3((unsigned)lm_blocks(h, a.g)), dim3(LM_THREADS), 0,
+    hipLaunchKernelGGL(low_mach_div_kernel, dim3((unsigned)walk_blocks(a.g)), dim3(LM_THREADS), 0,
                        h->stream, a);
     HIP_TRY(hipGetLastError());
   }
