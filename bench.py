@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--gate-dt-max", type=float, default=None, help="K")
     ap.add_argument("--gate-dt-freeze", type=float, default=None, help="K")
     ap.add_argument("--gate-dy-freeze", type=float, default=None)
+    ap.add_argument("--gate-rate-tol", type=float, default=None,
+                    help="mass fraction; 0 turns the rate-consistency test off")
     ap.add_argument("--table", type=int, default=1,
                     help="time the table_approximated mode beside the FP64 integrator (0: skip)")
     ap.add_argument("--low-mach", type=int, default=1,
@@ -506,10 +508,10 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_
     if args.gate_elem_floor is not None:
         gkw["elem_floor"] = args.gate_elem_floor
     for k, v in (("dT_max", args.gate_dt_max), ("dT_freeze", args.gate_dt_freeze),
-                 ("dY_freeze", args.gate_dy_freeze)):
+                 ("dY_freeze", args.gate_dy_freeze), ("rate_tol", args.gate_rate_tol)):
         if v is not None:
             gkw[k] = v
-    for k in ("dT_max", "dT_freeze", "dY_freeze", "energy_tol", "elem_tol", "elem_floor"):
+    for k in ("dT_max", "dT_freeze", "dY_freeze", "energy_tol", "elem_tol", "elem_floor", "rate_tol"):
         if k not in gkw and ("gate_" + k) in z.files:
             gkw[k] = float(z["gate_" + k])
     gate = sg.gate_struct(mech, **gkw)
@@ -548,7 +550,7 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_
         "fallback_nstep_max": max(r.fallback.nstep_max for r in recs),
         "rejects_per_step": {k: mean(lambda r, k=k: getattr(r, k)) for k in
                              ("reject_domain", "reject_confidence", "reject_elements",
-                              "reject_negative", "reject_energy")},
+                              "reject_negative", "reject_energy", "reject_rate")},
         "gate": gkw,
         "infer_ms": round(infer_ms, 3), "gate_ms": round(mean(lambda r: r.gate_ms), 3),
         "fallback_ms": round(mean(lambda r: r.fallback_ms), 3),

@@ -22,6 +22,12 @@
  *        - the network's |dT| is at most dT_max (confidence: fast chemistry,
  *          where a wrong reaction progress conserves elements and energy and
  *          so passes the checks below, goes to the FP64 integrator),
+ *        - with rate_tol > 0, the answer (network or frozen) agrees with the
+ *          explicit estimate of the step, C_s(t+dt) - C_s = omega_s(C, T) dt
+ *          from the FP64 rate constants and mass action at the cell's own
+ *          state, to rate_tol in every active species' mass fraction (a
+ *          state outside the training data that the network calls nearly
+ *          inert, such as radical-rich burnt gas recombining, fails it);
  *        - the raw dY conserves every element to elem_tol (relative to the
  *          atoms it moves) + elem_floor [mol/kg],
  *        - after projecting dY onto the element-conserving subspace, no
@@ -68,7 +74,9 @@ typedef struct nrg_surrogate_gate {
   double dT_max;                                     /* K                              */
   double dT_freeze, dY_freeze;                       /* K, mass fraction (dT_freeze 0: off) */
   int32_t n_elements;                                /* rows of `elements` (<= 4)      */
-  int32_t reserved[7];
+  int32_t reserved0;
+  double rate_tol;                                   /* mass fraction (0: off), below  */
+  int32_t reserved[4];
   double elements[NRG_SUR_MAX_ELEMENTS][32];         /* atoms of element e in species s */
 } nrg_surrogate_gate;
 
@@ -80,6 +88,7 @@ typedef struct nrg_surrogate_stats {
   double gate_ms;
   double fallback_ms;         /* FP64 integration of the rejected cells                */
   nrg_chem_stats fallback;    /* counters of that integration                          */
+  int64_t reject_rate;        /* rejected by the rate-consistency test (rate_tol)      */
 } nrg_surrogate_stats;
 
 /*

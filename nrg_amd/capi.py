@@ -141,7 +141,8 @@ class NrgSurrogateGate(C.Structure):
                 ("elem_tol", C.c_double), ("elem_floor", C.c_double), ("y_neg_tol", C.c_double),
                 ("energy_tol", C.c_double), ("dT_max", C.c_double),
                 ("dT_freeze", C.c_double), ("dY_freeze", C.c_double),
-                ("n_elements", C.c_int32), ("reserved", C.c_int32 * 7),
+                ("n_elements", C.c_int32), ("reserved0", C.c_int32), ("rate_tol", C.c_double),
+                ("reserved", C.c_int32 * 4),
                 ("elements", (C.c_double * 32) * NRG_SUR_MAX_ELEMENTS)]
 
 
@@ -153,7 +154,7 @@ class NrgSurrogateStats(C.Structure):
                 ("reject_negative", C.c_int64), ("reject_energy", C.c_int64),
                 ("reject_confidence", C.c_int64), ("cells_frozen", C.c_int64),
                 ("infer_ms", C.c_double), ("gate_ms", C.c_double), ("fallback_ms", C.c_double),
-                ("fallback", NrgChemStats)]
+                ("fallback", NrgChemStats), ("reject_rate", C.c_int64)]
 
 
 class NrgChemTuneTrial(C.Structure):

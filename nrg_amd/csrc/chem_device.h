@@ -359,20 +359,23 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
 // Mass-action right-hand side, the accumulation order of fun (:507-536), on
 // the active species (an inert species' f is zero).
 // ---------------------------------------------------------------------------
+// kl holds k_f(r) at kl[r*KS] and k_r(r) at kl[(R+r)*KS] (KS = WAVE: the
+// integrator's lane-interleaved LDS copy; KS = 1: a plain per-thread array).
+template <int KS = WAVE>
 NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[SA], double (&Zd)[SA])
 {
 #pragma unroll
   for (int s = 0; s < SA; ++s) Zd[s] = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    double fwd = kl[r * WAVE];
+    double fwd = kl[r * KS];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int sp = topo::left[r][c];
       if (c < topo::nl[r])
         fwd = fwd * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[act_of(sp)]);
     }
-    double rev = kl[(R + r) * WAVE];
+    double rev = kl[(R + r) * KS];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int sp = topo::right[r][c];

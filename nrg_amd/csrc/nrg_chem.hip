@@ -915,8 +915,8 @@ __global__ void __launch_bounds__(BIN_THREADS) bin_scatter_kernel(BinArgs a)
 constexpr int NIN = S + 3, NOUT = S + 1;
 
 enum { SUR_FALLBACK = 0, SUR_ACCEPT, SUR_GATED, SUR_R_DOMAIN, SUR_R_ELEM, SUR_R_NEG, SUR_R_ENERGY,
-       SUR_R_CONF, SUR_FROZEN, SUR_SLOTS = 9 };
-constexpr int SUR_NREASONS = SUR_R_CONF - SUR_R_DOMAIN + 1;
+       SUR_R_CONF, SUR_R_RATE, SUR_FROZEN, SUR_SLOTS = 10 };
+constexpr int SUR_NREASONS = SUR_R_RATE - SUR_R_DOMAIN + 1;
 
 struct SurArgs {
   GridIdx g;
@@ -929,7 +929,7 @@ struct SurArgs {
   float y_floor;
   double t_min, t_max, p_min, p_max;
   int dt_ok;
-  double elem_tol, elem_floor, y_neg_tol, energy_tol, dT_max, dT_freeze, dY_freeze;
+  double elem_tol, elem_floor, y_neg_tol, energy_tol, dT_max, dT_freeze, dY_freeze, rate_tol;
   int ne;
   double E[NRG_SUR_MAX_ELEMENTS][S];
   bf16 *X;                    // ninner x MLP_IN_PAD
@@ -940,6 +940,34 @@ struct SurArgs {
   int32_t *list;              // fallback cells (inner index)
   unsigned long long *acc;    // SUR_* counters
 };
+
+// The explicit estimate of one step, C_s(t+dt) - C_s = omega_s(C, T) dt, from
+// the FP64 rate constants and mass action at the cell's own state, against the
+// answer the gate would give (dy, or zero for a frozen cell): true if they
+// agree to tol in every active species' mass fraction.
+NRG_DI bool rate_consistent(const MechCoef &mc, double T, double rho, const double (&y)[S],
+                            const double (&dy)[S], bool frozen, double dt, double tol)
+{
+  double C[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) C[s] = (y[s] * rho) / mc.W[s];
+  double k[2 * R + NM];
+  rate_constants<InlineMath>(mc, T, C, k, 1);
+  double Mc[NM], Z[SA], Zd[SA];
+#pragma unroll
+  for (int j = 0; j < NM; ++j) Mc[j] = k[2 * R + j];
+#pragma unroll
+  for (int i = 0; i < SA; ++i) Z[i] = C[kSp.full[i]];
+  rhs<1>(k, Mc, Z, Zd);
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < SA; ++i) {
+    const int s = kSp.full[i];
+    const double est = ((Zd[i] * dt) * mc.W[s]) / rho;
+    ok = ok && fabs(est - (frozen ? 0.0 : dy[s])) <= tol;
+  }
+  return ok;
+}
 
 NRG_DI bool gated_cell(const SurArgs &a, long long off)
 {
@@ -1031,6 +1059,7 @@ NRG_DI bool small_solve(double (&M)[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS],
 
 // The per-reason counts are summed per block in LDS and added to the global
 // counters once per block.
+template <bool RATE>
 __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 {
   __shared__ MechCoef mc;
@@ -1066,6 +1095,10 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
       for (int s = 0; s < S; ++s) dymax = fmax(dymax, fabs(dy[s]));
       frozen = a.dT_freeze > 0.0 && fabs(dT) <= a.dT_freeze && dymax <= a.dY_freeze;
       if (!frozen && !(fabs(dT) <= a.dT_max)) reason = SUR_R_CONF;
+      if (RATE && reason == 0 && !rate_consistent(mc, T, rho, y, dy, frozen, a.dt, a.rate_tol)) {
+        frozen = false;
+        reason = SUR_R_RATE;
+      }
     }
     if (frozen) {
       // inert: the FP64 step's outputs for a cell whose concentrations do not change
@@ -2389,6 +2422,8 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   sa.elem_tol = gate->elem_tol; sa.elem_floor = gate->elem_floor;
   sa.y_neg_tol = gate->y_neg_tol; sa.energy_tol = gate->energy_tol; sa.dT_max = gate->dT_max;
   sa.dT_freeze = gate->dT_freeze; sa.dY_freeze = gate->dY_freeze;
+  if (!(gate->rate_tol >= 0.0)) return set_error(NRG_CHEM_E_ARG, "gate rate_tol must be >= 0");
+  sa.rate_tol = gate->rate_tol;
   sa.ne = gate->n_elements;
   for (int e = 0; e < sa.ne; ++e)
     for (int s = 0; s < S; ++s) sa.E[e][s] = gate->elements[e][s];
@@ -2422,7 +2457,10 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     HIP_TRY(hipGetLastError());
     if ((st = mlp_infer_bf16(mlp, n, h->sur_X, h->sur_Y, h->stream))) return st;
     HIP_TRY(hipEventRecord(h->ev2, h->stream));
-    hipLaunchKernelGGL(gate_kernel, dim3(nb), dim3(256), 0, h->stream, sa);
+    if (sa.rate_tol > 0.0)
+      hipLaunchKernelGGL(gate_kernel<true>, dim3(nb), dim3(256), 0, h->stream, sa);
+    else
+      hipLaunchKernelGGL(gate_kernel<false>, dim3(nb), dim3(256), 0, h->stream, sa);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev3, h->stream));
     HIP_TRY(hipMemcpyAsync(sur, h->sur_acc, sizeof(sur), hipMemcpyDeviceToHost, h->stream));
@@ -2460,6 +2498,7 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     stats->reject_negative = (int64_t)sur[SUR_R_NEG];
     stats->reject_energy = (int64_t)sur[SUR_R_ENERGY];
     stats->reject_confidence = (int64_t)sur[SUR_R_CONF];
+    stats->reject_rate = (int64_t)sur[SUR_R_RATE];
     stats->cells_frozen = (int64_t)sur[SUR_FROZEN];
     stats->infer_ms = ms_inf;
     stats->gate_ms = ms_gate;

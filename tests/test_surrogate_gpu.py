@@ -201,3 +201,32 @@ def test_fallback_in_stiffness_order_is_bitwise_identical(engine, mech, zero_net
             assert a["stats"].fallback.nfe_sum == b["stats"].fallback.nfe_sum
     finally:
         eng.close()
+
+
+def test_rate_consistency_rejects_what_the_explicit_step_contradicts(engine, oracle, mech, zero_net):
+    """rate_tol: with a network that predicts no change, a cell is kept only if
+    the explicit estimate of its step, omega(C, T) dt W_s / rho from the oracle's
+    rate constants and mass action, stays within rate_tol in every species;
+    the rejected cells are the FP64 step's, bit for bit."""
+    st, bc = _state(mech, seed=12)
+    tol, dt = 1e-6, 1e-7
+    o = _run(engine, zero_net, _norm(np.zeros(S + 1)), _gate(mech, rate_tol=tol), st, bc, dt)
+    f = _fp64(engine, st, bc, dt)
+    T, rho, Y = st["T"].cpu().numpy(), st["rho"].cpu().numpy(), st["Y"].cpu().numpy()
+    live = (T >= 305.0) & (bc.cpu().numpy() == 0)
+    W = np.asarray(mech.molar_masses)
+    est = np.zeros(T.size)
+    for q in np.flatnonzero(live):
+        Mc = oracle.mixture_concentration(Y[:, q], rho[q])
+        kf, kr = oracle.rate_constants(T[q], Mc)
+        w = oracle.rhs(kf, kr, Mc, Y[:, q] * rho[q] / W)
+        est[q] = np.abs(w * dt * W / rho[q]).max()
+    s = o["stats"]
+    rej = live & (o["status"] != capi.NRG_CELL_SURROGATE)
+    _report("rate_consistency", rejected=int(rej.sum()), live=int(live.sum()),
+            reject_rate=s.reject_rate)
+    assert s.reject_rate == rej.sum() == s.cells_fallback and 0 < rej.sum() < live.sum()
+    assert np.all(rej[live & (est > 2.0 * tol)]) and not np.any(rej[live & (est < 0.5 * tol)])
+    for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+        assert np.array_equal(o[k][..., rej], f[k][..., rej]), k
+    assert np.all(o["Y_prod"][:, live & ~rej] == 0)
