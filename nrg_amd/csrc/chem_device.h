@@ -275,6 +275,78 @@ NRG_DI double e_molar(const MechCoef &mc, double T, int s)
 // reactions whose RHS term uses it, kout[(2R+slot)*stride] (:235-248).
 // ---------------------------------------------------------------------------
 // Reactions [R0, R1) only (their k_f, k_r and [M] slots).
+// k_f, k_r and [M] of reaction r at T (already capped at 1e4 K), from the
+// species' entropies sv and enthalpies hv at T (:394-485).
+template <class M>
+NRG_DI void reaction_k(const MechCoef &mc, double T, const double (&sv)[S], const double (&hv)[S],
+                       const double (&C)[S], int r, double &kf_out, double &kr_out, double &m_out)
+{
+  double m = 0.0;
+  if (uses_m(r)) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) m = m + C[s] * mc.eff[r][s];
+  }
+  double d_s = 0.0, d_h = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int sp = topo::right[r][c];
+    if (c < topo::nr[r] && sp != S) {
+      d_s = d_s + sv[sp < S ? sp : 0];
+      d_h = d_h + hv[sp < S ? sp : 0];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int sp = topo::left[r][c];
+    if (c < topo::nl[r] && sp != S) {
+      d_s = d_s - sv[sp < S ? sp : 0];
+      d_h = d_h - hv[sp < S ? sp : 0];
+    }
+  }
+  const double d_nu = (double)(topo::nr[r] - topo::nl[r]);
+  const int rt = topo::rtype[r];
+  double kf = 0.0, kr = 0.0;
+  if (rt == 0 || rt == 1 || rt == 5 || rt == 6) {
+    kf = (mc.A[r] * M::pow_(T, mc.beta[r])) * M::exp_(mc.mE_R[r] / T);
+  } else if (rt == 2 || rt == 7) {
+    const double k0 = (mc.A_low[r] * M::pow_(T, mc.beta_low[r])) * M::exp_(mc.mElow_R[r] / T);
+    const double kinf = (mc.A[r] * M::pow_(T, mc.beta[r])) * M::exp_(mc.mE_R[r] / T);
+    const double pr = (k0 * m) / kinf;
+    kf = kinf * (pr / (1.0 + pr));
+  } else if (rt == 3) {
+    double k0 = M::exp_(mc.mElow_R[r] / T);
+    double kinf = M::exp_(mc.mE_R[r] / T);
+    k0 = (mc.A_low[r] * M::pow_(T, mc.beta_low[r])) * k0;
+    kinf = (mc.A[r] * M::pow_(T, mc.beta[r])) * kinf;
+    const double pr = (k0 * m) / kinf;
+    const double fcent = (mc.one_m_alpha[r] * M::exp_(-(T / mc.troe[1][r])) +
+                          mc.troe[0][r] * M::exp_(-(T / mc.troe[2][r]))) +
+                         M::exp_(-(mc.troe[3][r] / T));
+    const double lf = M::log10_(fcent);
+    const double c = -0.4 - 0.67 * lf;
+    const double n = 0.75 - 1.27 * lf;
+    const double d = 0.14;
+    double blend;
+    if (pr != 0.0) {
+      const double lp = M::log10_(pr);
+      const double x = (lp + c) / (n - d * (lp + c));
+      blend = M::pow_(fcent, 1.0 / (1.0 + x * x));
+    } else {
+      blend = M::pow_(fcent, -1.0 / d);
+    }
+    kf = (kinf * (pr / (1.0 + pr))) * blend;
+  }
+  if (rt == 0 || rt == 1 || rt == 2 || rt == 3) {
+    const double Kp = M::exp_(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
+    // K_c = K_p * ((P_atm/R_u)/T)**d_nu; d_nu = 0 gives pow(x, 0) = 1 exactly
+    const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * M::pow_(mc.patm_r / T, d_nu);
+    kr = kf / Kc;
+  }
+  kf_out = kf;
+  kr_out = kr;
+  m_out = m;
+}
+
 template <class M, int R0 = 0, int R1 = R>
 NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], double *kout,
                            long long stride)
@@ -288,68 +360,9 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
   }
 #pragma unroll
   for (int r = R0; r < R1; ++r) {
-    double m = 0.0;
-    if (uses_m(r)) {
-#pragma unroll
-      for (int s = 0; s < S; ++s) m = m + C[s] * mc.eff[r][s];
-      if (kMSlot.v[r] >= 0) kout[(2 * R + kMSlot.v[r]) * stride] = m;
-    }
-    double d_s = 0.0, d_h = 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int sp = topo::right[r][c];
-      if (c < topo::nr[r] && sp != S) {
-        d_s = d_s + sv[sp < S ? sp : 0];
-        d_h = d_h + hv[sp < S ? sp : 0];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int sp = topo::left[r][c];
-      if (c < topo::nl[r] && sp != S) {
-        d_s = d_s - sv[sp < S ? sp : 0];
-        d_h = d_h - hv[sp < S ? sp : 0];
-      }
-    }
-    const double d_nu = (double)(topo::nr[r] - topo::nl[r]);
-    const int rt = topo::rtype[r];
-    double kf = 0.0, kr = 0.0;
-    if (rt == 0 || rt == 1 || rt == 5 || rt == 6) {
-      kf = (mc.A[r] * M::pow_(T, mc.beta[r])) * M::exp_(mc.mE_R[r] / T);
-    } else if (rt == 2 || rt == 7) {
-      const double k0 = (mc.A_low[r] * M::pow_(T, mc.beta_low[r])) * M::exp_(mc.mElow_R[r] / T);
-      const double kinf = (mc.A[r] * M::pow_(T, mc.beta[r])) * M::exp_(mc.mE_R[r] / T);
-      const double pr = (k0 * m) / kinf;
-      kf = kinf * (pr / (1.0 + pr));
-    } else if (rt == 3) {
-      double k0 = M::exp_(mc.mElow_R[r] / T);
-      double kinf = M::exp_(mc.mE_R[r] / T);
-      k0 = (mc.A_low[r] * M::pow_(T, mc.beta_low[r])) * k0;
-      kinf = (mc.A[r] * M::pow_(T, mc.beta[r])) * kinf;
-      const double pr = (k0 * m) / kinf;
-      const double fcent = (mc.one_m_alpha[r] * M::exp_(-(T / mc.troe[1][r])) +
-                            mc.troe[0][r] * M::exp_(-(T / mc.troe[2][r]))) +
-                           M::exp_(-(mc.troe[3][r] / T));
-      const double lf = M::log10_(fcent);
-      const double c = -0.4 - 0.67 * lf;
-      const double n = 0.75 - 1.27 * lf;
-      const double d = 0.14;
-      double blend;
-      if (pr != 0.0) {
-        const double lp = M::log10_(pr);
-        const double x = (lp + c) / (n - d * (lp + c));
-        blend = M::pow_(fcent, 1.0 / (1.0 + x * x));
-      } else {
-        blend = M::pow_(fcent, -1.0 / d);
-      }
-      kf = (kinf * (pr / (1.0 + pr))) * blend;
-    }
-    if (rt == 0 || rt == 1 || rt == 2 || rt == 3) {
-      const double Kp = M::exp_(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
-      // K_c = K_p * ((P_atm/R_u)/T)**d_nu; d_nu = 0 gives pow(x, 0) = 1 exactly
-      const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * M::pow_(mc.patm_r / T, d_nu);
-      kr = kf / Kc;
-    }
+    double kf, kr, m;
+    reaction_k<M>(mc, T, sv, hv, C, r, kf, kr, m);
+    if (uses_m(r) && kMSlot.v[r] >= 0) kout[(2 * R + kMSlot.v[r]) * stride] = m;
     kout[r * stride] = kf;
     kout[(R + r) * stride] = kr;
   }
@@ -359,39 +372,69 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
 // Mass-action right-hand side, the accumulation order of fun (:507-536), on
 // the active species (an inert species' f is zero).
 // ---------------------------------------------------------------------------
+// Mass-action contribution of reaction r (:520-535) with rate constants kf,
+// kr and third-body concentration m.
+NRG_DI void rhs_reaction(int r, double kf, double kr, double m, const double (&Z)[SA], double (&Zd)[SA])
+{
+  double fwd = kf;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int sp = topo::left[r][c];
+    if (c < topo::nl[r]) fwd = fwd * ((sp == S) ? m : Z[act_of(sp)]);
+  }
+  double rev = kr;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int sp = topo::right[r][c];
+    if (c < topo::nr[r]) rev = rev * ((sp == S) ? m : Z[act_of(sp)]);
+  }
+#pragma unroll
+  for (int a = 0; a < SA; ++a) {
+    const int s = kSp.full[a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (c < topo::nl[r] && topo::left[r][c] == s) Zd[a] = (Zd[a] - fwd) + rev;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (c < topo::nr[r] && topo::right[r][c] == s) Zd[a] = (Zd[a] - rev) + fwd;
+  }
+}
+
 // kl holds k_f(r) at kl[r*KS] and k_r(r) at kl[(R+r)*KS] (KS = WAVE: the
-// integrator's lane-interleaved LDS copy; KS = 1: a plain per-thread array).
+// integrator's lane-interleaved LDS copy).
 template <int KS = WAVE>
 NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[SA], double (&Zd)[SA])
 {
 #pragma unroll
   for (int s = 0; s < SA; ++s) Zd[s] = 0.0;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    double fwd = kl[r * KS];
+  for (int r = 0; r < R; ++r)
+    rhs_reaction(r, kl[r * KS], kl[(R + r) * KS], Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0], Z, Zd);
+}
+
+// The part of omega(C, T) of the active species that reactions [R0, R1)
+// contribute, with the rate constants computed one reaction at a time (no
+// array of all k_f, k_r is held).
+template <class Mth, int R0 = 0, int R1 = R>
+NRG_DI void explicit_rates(const MechCoef &mc, double Tin, const double (&C)[S], double (&Zd)[SA])
+{
+  const double T = dmin(Tin, 10000.0);
+  double sv[S], hv[S], Z[SA];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int sp = topo::left[r][c];
-      if (c < topo::nl[r])
-        fwd = fwd * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[act_of(sp)]);
-    }
-    double rev = kl[(R + r) * KS];
+  for (int c = 0; c < S; ++c) {
+    sv[c] = s_molar<Mth>(mc, T, c);
+    hv[c] = h_molar(mc, T, c);
+  }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int sp = topo::right[r][c];
-      if (c < topo::nr[r])
-        rev = rev * ((sp == S) ? Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0] : Z[act_of(sp)]);
-    }
+  for (int i = 0; i < SA; ++i) {
+    Z[i] = C[kSp.full[i]];
+    Zd[i] = 0.0;
+  }
 #pragma unroll
-    for (int a = 0; a < SA; ++a) {
-      const int s = kSp.full[a];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        if (c < topo::nl[r] && topo::left[r][c] == s) Zd[a] = (Zd[a] - fwd) + rev;
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        if (c < topo::nr[r] && topo::right[r][c] == s) Zd[a] = (Zd[a] - rev) + fwd;
-    }
+  for (int r = R0; r < R1; ++r) {
+    double kf, kr, m;
+    reaction_k<Mth>(mc, T, sv, hv, C, r, kf, kr, m);
+    rhs_reaction(r, kf, kr, m, Z, Zd);
   }
 }
 

@@ -938,36 +938,9 @@ struct SurArgs {
   double *Efp;
   int32_t *status;
   int32_t *list;              // fallback cells (inner index)
+  int8_t *rate_bad;           // per inner cell: 1 = the rate-consistency test failed
   unsigned long long *acc;    // SUR_* counters
 };
-
-// The explicit estimate of one step, C_s(t+dt) - C_s = omega_s(C, T) dt, from
-// the FP64 rate constants and mass action at the cell's own state, against the
-// answer the gate would give (dy, or zero for a frozen cell): true if they
-// agree to tol in every active species' mass fraction.
-NRG_DI bool rate_consistent(const MechCoef &mc, double T, double rho, const double (&y)[S],
-                            const double (&dy)[S], bool frozen, double dt, double tol)
-{
-  double C[S];
-#pragma unroll
-  for (int s = 0; s < S; ++s) C[s] = (y[s] * rho) / mc.W[s];
-  double k[2 * R + NM];
-  rate_constants<InlineMath>(mc, T, C, k, 1);
-  double Mc[NM], Z[SA], Zd[SA];
-#pragma unroll
-  for (int j = 0; j < NM; ++j) Mc[j] = k[2 * R + j];
-#pragma unroll
-  for (int i = 0; i < SA; ++i) Z[i] = C[kSp.full[i]];
-  rhs<1>(k, Mc, Z, Zd);
-  bool ok = true;
-#pragma unroll
-  for (int i = 0; i < SA; ++i) {
-    const int s = kSp.full[i];
-    const double est = ((Zd[i] * dt) * mc.W[s]) / rho;
-    ok = ok && fabs(est - (frozen ? 0.0 : dy[s])) <= tol;
-  }
-  return ok;
-}
 
 NRG_DI bool gated_cell(const SurArgs &a, long long off)
 {
@@ -1059,6 +1032,67 @@ NRG_DI bool small_solve(double (&M)[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS],
 
 // The per-reason counts are summed per block in LDS and added to the global
 // counters once per block.
+// The rate-consistency test of the gate, in a kernel of its own (the gate
+// kernel's registers leave no room for the rate constants).  For every live
+// cell the answer the gate would give before its conservation checks (the
+// network's increment, or zero for a cell it freezes) is compared with the
+// explicit estimate of the step, C_s(t+dt) - C_s = omega_s(C, T) dt, from the
+// FP64 rate constants and mass action at the cell's own state;
+// rate_bad[q] = 1 where some active species differs by more than rate_tol in
+// mass fraction.  As in rates_kernel, waves 0-1 and 2-3 of a block take the
+// two halves of the reactions for the same 128 cells (one half per thread
+// fits in half the registers) and add their parts of omega in LDS.
+constexpr int RCHECK_CELLS = 128;
+
+__global__ void __launch_bounds__(256) rate_check_kernel(SurArgs a)
+{
+  __shared__ MechCoef mc;
+  __shared__ double part[RCHECK_CELLS][SA];
+  stage_mech(mc, a.mech);
+  const int group = __builtin_amdgcn_readfirstlane(threadIdx.x / RCHECK_CELLS);
+  const int local = threadIdx.x % RCHECK_CELLS;
+  const long long q = (long long)blockIdx.x * RCHECK_CELLS + local;
+  const long long off = q < a.g.ninner ? cell_offset(a.g, q) : 0;
+  const bool live = q < a.g.ninner && !gated_cell(a, off);
+  double y[S], C[S], Zd[SA];
+  double rho = 1.0;
+  if (live) {
+    rho = a.rho[off];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      y[s] = a.Y[s][off];
+      C[s] = (y[s] * rho) / mc.W[s];
+    }
+    if (group == 0) explicit_rates<InlineMath, 0, R / 2>(mc, a.T[off], C, Zd);
+    else explicit_rates<InlineMath, R / 2, R>(mc, a.T[off], C, Zd);
+    if (group == 1) {
+#pragma unroll
+      for (int i = 0; i < SA; ++i) part[local][i] = Zd[i];
+    }
+  }
+  __syncthreads();
+  if (group != 0 || q >= a.g.ninner) return;
+  int8_t bad = 0;
+  if (live) {
+    const float *yp = a.Ypred + q * NOUT;
+    const double dT = (double)yp[0] * a.out_scale[0] + a.out_shift[0];
+    double dy[S], dymax = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      dy[s] = (double)yp[1 + s] * a.out_scale[1 + s] + a.out_shift[1 + s];
+      dymax = fmax(dymax, fabs(dy[s]));
+    }
+    const bool frozen = a.dT_freeze > 0.0 && fabs(dT) <= a.dT_freeze && dymax <= a.dY_freeze;
+#pragma unroll
+    for (int i = 0; i < SA; ++i) {
+      const int s = kSp.full[i];
+      const double est = (((Zd[i] + part[local][i]) * a.dt) * mc.W[s]) / rho;
+      if (!(fabs(est - (frozen ? 0.0 : dy[s])) <= a.rate_tol)) bad = 1;
+    }
+  }
+  a.rate_bad[q] = bad;
+}
+
 template <bool RATE>
 __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 {
@@ -1095,7 +1129,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
       for (int s = 0; s < S; ++s) dymax = fmax(dymax, fabs(dy[s]));
       frozen = a.dT_freeze > 0.0 && fabs(dT) <= a.dT_freeze && dymax <= a.dY_freeze;
       if (!frozen && !(fabs(dT) <= a.dT_max)) reason = SUR_R_CONF;
-      if (RATE && reason == 0 && !rate_consistent(mc, T, rho, y, dy, frozen, a.dt, a.rate_tol)) {
+      if (RATE && reason == 0 && a.rate_bad[q]) {
         frozen = false;
         reason = SUR_R_RATE;
       }
@@ -1271,6 +1305,7 @@ struct nrg_chem_handle {
   nrg::bf16 *sur_X = nullptr;
   float *sur_Y = nullptr;
   int32_t *sur_list = nullptr;
+  int8_t *sur_rate_bad = nullptr;
   unsigned long long *sur_acc = nullptr;
   long long sur_cells = 0;
   // stiffness-ordered queue
@@ -1599,7 +1634,8 @@ void nrg_chem_destroy(nrg_chem_t *h)
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->ev2) hipEventDestroy(h->ev2);
   if (h->ev3) hipEventDestroy(h->ev3);
-  for (void *p : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_acc,
+  for (void *p : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_rate_bad,
+                  (void *)h->sur_acc,
                   (void *)h->order_list, (void *)h->hist, (void *)h->bins, (void *)h->table})
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -2431,18 +2467,21 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   HIP_TRY(hipSetDevice(h->device));
   const long long n = a.g.ninner;
   if (n > h->sur_cells) {
-    for (void *q : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list})
+    for (void *q : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_rate_bad})
       if (q) HIP_TRY(hipFree(q));
-    h->sur_X = nullptr; h->sur_Y = nullptr; h->sur_list = nullptr; h->sur_cells = 0;
+    h->sur_X = nullptr; h->sur_Y = nullptr; h->sur_list = nullptr; h->sur_rate_bad = nullptr;
+    h->sur_cells = 0;
     HIP_TRY(hipMalloc(&h->sur_X, (size_t)n * MLP_IN_PAD * sizeof(bf16)));
     HIP_TRY(hipMalloc(&h->sur_Y, (size_t)n * NOUT * sizeof(float)));
     HIP_TRY(hipMalloc(&h->sur_list, (size_t)n * sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&h->sur_rate_bad, (size_t)n * sizeof(int8_t)));
     h->sur_cells = n;
   }
   if (!h->sur_acc) HIP_TRY(hipMalloc(&h->sur_acc, SUR_SLOTS * sizeof(unsigned long long)));
   sa.X = h->sur_X;
   sa.Ypred = h->sur_Y;
   sa.list = h->sur_list;
+  sa.rate_bad = h->sur_rate_bad;
   sa.acc = h->sur_acc;
 
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
@@ -2457,9 +2496,11 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     HIP_TRY(hipGetLastError());
     if ((st = mlp_infer_bf16(mlp, n, h->sur_X, h->sur_Y, h->stream))) return st;
     HIP_TRY(hipEventRecord(h->ev2, h->stream));
-    if (sa.rate_tol > 0.0)
+    if (sa.rate_tol > 0.0) {
+      hipLaunchKernelGGL(rate_check_kernel, dim3((unsigned)((n + RCHECK_CELLS - 1) / RCHECK_CELLS)),
+                         dim3(256), 0, h->stream, sa);
       hipLaunchKernelGGL(gate_kernel<true>, dim3(nb), dim3(256), 0, h->stream, sa);
-    else
+    } else
       hipLaunchKernelGGL(gate_kernel<false>, dim3(nb), dim3(256), 0, h->stream, sa);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev3, h->stream));
