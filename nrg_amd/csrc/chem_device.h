@@ -118,6 +118,23 @@ struct InlineMath {
   static NRG_DI double log10_(double x) { return log10(x); }
 };
 
+// Cheaper transcendentals for estimates that no parity path depends on (the
+// surrogate gate's rate-consistency test): x^y as exp(y ln x) on positive x,
+// so ln T and ln(P_atm/(R_u T)) are computed once per cell and shared by every
+// reaction instead of a full pow per rate constant.  Relative error ~1e-15
+// for the exponents of a mechanism (|y ln x| < 50).
+struct EstimateMath {
+  static NRG_DI double pow_(double x, double y)
+  {
+    if (y == 0.0) return 1.0;
+    if (y == 1.0) return x;
+    return exp(y * log(x));
+  }
+  static NRG_DI double exp_(double x) { return exp(x); }
+  static NRG_DI double log_(double x) { return log(x); }
+  static NRG_DI double log10_(double x) { return log10(x); }
+};
+
 // A value pinned in a VGPR.  Select chains over register arrays go through it,
 // so the optimiser cannot turn "select between loads" into one load through a
 // selected address (which would keep the whole array in private memory).

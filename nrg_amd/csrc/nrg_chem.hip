@@ -1037,7 +1037,8 @@ NRG_DI bool small_solve(double (&M)[NRG_SUR_MAX_ELEMENTS][NRG_SUR_MAX_ELEMENTS],
 // cell the answer the gate would give before its conservation checks (the
 // network's increment, or zero for a cell it freezes) is compared with the
 // explicit estimate of the step, C_s(t+dt) - C_s = omega_s(C, T) dt, from the
-// FP64 rate constants and mass action at the cell's own state;
+// FP64 rate constants (EstimateMath transcendentals) and mass action at the
+// cell's own state;
 // rate_bad[q] = 1 where some active species differs by more than rate_tol in
 // mass fraction.  As in rates_kernel, waves 0-1 and 2-3 of a block take the
 // two halves of the reactions for the same 128 cells (one half per thread
@@ -1063,8 +1064,8 @@ __global__ void __launch_bounds__(256) rate_check_kernel(SurArgs a)
       y[s] = a.Y[s][off];
       C[s] = (y[s] * rho) / mc.W[s];
     }
-    if (group == 0) explicit_rates<InlineMath, 0, R / 2>(mc, a.T[off], C, Zd);
-    else explicit_rates<InlineMath, R / 2, R>(mc, a.T[off], C, Zd);
+    if (group == 0) explicit_rates<EstimateMath, 0, R / 2>(mc, a.T[off], C, Zd);
+    else explicit_rates<EstimateMath, R / 2, R>(mc, a.T[off], C, Zd);
     if (group == 1) {
 #pragma unroll
       for (int i = 0; i < SA; ++i) part[local][i] = Zd[i];
