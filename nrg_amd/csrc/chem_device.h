@@ -116,23 +116,34 @@ struct InlineMath {
   }
   static NRG_DI double log_(double x) { return log(x); }
   static NRG_DI double log10_(double x) { return log10(x); }
+  // A T^b exp(mE_R / T), in the reference's order of operations
+  static NRG_DI double arrh_(double A, double b, double mE_R, double T)
+  {
+    return (A * pow_(T, b)) * exp_(mE_R / T);
+  }
 };
 
 // Cheaper transcendentals for estimates that no parity path depends on (the
 // surrogate gate's rate-consistency test): x^y as exp(y ln x) on positive x,
 // so ln T and ln(P_atm/(R_u T)) are computed once per cell and shared by every
-// reaction instead of a full pow per rate constant.  Relative error ~1e-15
-// for the exponents of a mechanism (|y ln x| < 50).
+// reaction instead of a full pow per rate constant, and an Arrhenius term as
+// one exp.  Relative error ~1e-15 for the exponents of a mechanism
+// (|y ln x| < 50).
 struct EstimateMath {
   static NRG_DI double pow_(double x, double y)
   {
     if (y == 0.0) return 1.0;
     if (y == 1.0) return x;
+    if (y == -1.0) return 1.0 / x;
     return exp(y * log(x));
   }
   static NRG_DI double exp_(double x) { return exp(x); }
   static NRG_DI double log_(double x) { return log(x); }
   static NRG_DI double log10_(double x) { return log10(x); }
+  static NRG_DI double arrh_(double A, double b, double mE_R, double T)
+  {
+    return A * exp(b * log(T) + mE_R / T);
+  }
 };
 
 // A value pinned in a VGPR.  Select chains over register arrays go through it,
@@ -324,17 +335,16 @@ NRG_DI void reaction_k(const MechCoef &mc, double T, const double (&sv)[S], cons
   const int rt = topo::rtype[r];
   double kf = 0.0, kr = 0.0;
   if (rt == 0 || rt == 1 || rt == 5 || rt == 6) {
-    kf = (mc.A[r] * M::pow_(T, mc.beta[r])) * M::exp_(mc.mE_R[r] / T);
+    kf = M::arrh_(mc.A[r], mc.beta[r], mc.mE_R[r], T);
   } else if (rt == 2 || rt == 7) {
-    const double k0 = (mc.A_low[r] * M::pow_(T, mc.beta_low[r])) * M::exp_(mc.mElow_R[r] / T);
-    const double kinf = (mc.A[r] * M::pow_(T, mc.beta[r])) * M::exp_(mc.mE_R[r] / T);
+    const double k0 = M::arrh_(mc.A_low[r], mc.beta_low[r], mc.mElow_R[r], T);
+    const double kinf = M::arrh_(mc.A[r], mc.beta[r], mc.mE_R[r], T);
     const double pr = (k0 * m) / kinf;
     kf = kinf * (pr / (1.0 + pr));
   } else if (rt == 3) {
-    double k0 = M::exp_(mc.mElow_R[r] / T);
-    double kinf = M::exp_(mc.mE_R[r] / T);
-    k0 = (mc.A_low[r] * M::pow_(T, mc.beta_low[r])) * k0;
-    kinf = (mc.A[r] * M::pow_(T, mc.beta[r])) * kinf;
+    // (A T^b) exp(-E/RT): the product the reference forms after its two exps
+    const double k0 = M::arrh_(mc.A_low[r], mc.beta_low[r], mc.mElow_R[r], T);
+    const double kinf = M::arrh_(mc.A[r], mc.beta[r], mc.mE_R[r], T);
     const double pr = (k0 * m) / kinf;
     const double fcent = (mc.one_m_alpha[r] * M::exp_(-(T / mc.troe[1][r])) +
                           mc.troe[0][r] * M::exp_(-(T / mc.troe[2][r]))) +
