@@ -1280,6 +1280,8 @@ struct StepKey {
   const double *rates;         // buffer the graph's kernels write (reallocated when it grows)
   int use_hist, form;
   long long chunk_max;
+  long long persistent_blocks; // grid of the persistent form (the autotuner may change it alone)
+  int jacobian;                // NRG_JAC_*: which integrator kernel the graph launches
 };
 
 struct nrg_chem_handle {
@@ -1799,6 +1801,8 @@ static int launch_step_graph(nrg_chem_t *h, StepArgs &a, bool use_hist)
   key.use_hist = use_hist;
   key.form = h->form;
   key.chunk_max = h->chunk_max;
+  key.persistent_blocks = h->persistent_blocks;
+  key.jacobian = h->opts.jacobian;
   if (!h->graph_exec || std::memcmp(&key, &h->graph_key, sizeof(key)) != 0) {
     if (h->graph_exec) HIP_TRY(hipGraphExecDestroy(h->graph_exec));
     h->graph_exec = nullptr;

@@ -367,6 +367,33 @@ class RefSlatec:
         for fn in ("dnrm2_", "dgefa_", "dgesl_", "ddcst_"):
             getattr(L, fn).restype = None
         L.dnrm2_.restype = C.c_double
+        vp = C.c_void_p
+        L.ref_chem_cells.argtypes = [vp, vp, C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp,
+                                     vp, vp, C.c_int]
+        L.ref_chem_cells.restype = C.c_int
+
+    def chem_cells(self, oracle, dt, T, rho, Y, nthreads=1):
+        """The chemistry step over a flat list of cells (no markers), each cell
+        integrated by the reference's DDRIV3 with the oracle's rate constants
+        and right-hand side; same outputs as Oracle.chem_cells."""
+        n = len(T)
+        S = oracle.S
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        yp = np.zeros((S, n))
+        yp2 = np.zeros((S, n))
+        ef = np.zeros(n)
+        st = np.zeros(n, dtype=np.int32)
+        ns, nf, nj = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        rc = self.lib.ref_chem_cells(oracle.mp, C.cast(C.pointer(oracle.opts), C.c_void_p), n, dt,
+                                     T.ctypes.data, rho.ctypes.data, _ptrs(list(Y)),
+                                     _ptrs(list(yp)), _ptrs(list(yp2)), ef.ctypes.data,
+                                     st.ctypes.data, ns.ctypes.data, nf.ctypes.data,
+                                     nj.ctypes.data, int(nthreads))
+        if rc != 0:
+            raise RuntimeError("ref_chem_cells failed (%d)" % rc)
+        return dict(Y_prod=yp, Y_prod2=yp2, E_f_prod=ef, status=st, nstep=ns, nfe=nf, nje=nj)
 
     def dnrm2(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
