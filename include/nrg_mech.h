@@ -43,6 +43,21 @@ int nrg_mech_load(const char *mechanism_path, const char *thermo_path,
                   const char *molar_masses_path, double default_enhanced_efficiencies,
                   const char *E_act_units, nrg_mech_file **out);
 
+/* Loads a task set-up folder the way NRG's main program does
+ * (computing_module/src/current_build/main.f90:170-171): the namelist
+ *   <dir>/chemical_data.inf         &chemical_properties chemical_mechanism_file_name,
+ *                                   default_enhanced_efficiencies, E_act_units /
+ * (package_library/src/chemical_properties_class.f90:77-97), the namelist
+ *   <dir>/thermophysical_setup.inf  &thermo_properties thermo_data_file_name,
+ *                                   transport_data_file_name, molar_masses_data_file_name /
+ * (package_library/src/thermophysical_properties_class.f90:97-134), then the
+ * named files <dir>/chemical_mechanisms/<mechanism> and
+ * <dir>/thermophysical_data/<thermo>, <dir>/thermophysical_data/<molar masses>
+ * through nrg_mech_load.  The transport file is not read (the chemistry step
+ * does not use it).  Names and groups are case-insensitive; quoted values lose
+ * their trailing blanks.  A missing file, group or variable is NRG_CHEM_E_FORMAT. */
+int nrg_mech_load_task_setup(const char *task_setup_dir, nrg_mech_file **out);
+
 /* Points *view at the loaded arrays (valid until nrg_mech_free), ready for
  * nrg_chem_create. */
 int nrg_mech_view(const nrg_mech_file *m, nrg_chem_mech *view);

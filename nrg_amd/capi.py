@@ -334,6 +334,8 @@ def load_library(path=None):
     lib.nrg_mech_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_char_p,
                                   C.POINTER(vp)]
     lib.nrg_mech_load.restype = C.c_int
+    lib.nrg_mech_load_task_setup.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.nrg_mech_load_task_setup.restype = C.c_int
     lib.nrg_mech_view.argtypes = [vp, C.POINTER(NrgChemMech)]
     lib.nrg_mech_view.restype = C.c_int
     lib.nrg_mech_species_name.argtypes = [vp, C.c_int32]

@@ -545,9 +545,163 @@ void parse_thermo(const char *thermo_path, const char *mm_path, nrg_mech_file &m
   }
 }
 
+// ---- task set-up: the namelist files that name the data files -------------
+//
+// NRG's main program builds the chemistry and the thermochemistry from two
+// namelist files of its task set-up folder (computing_module/src/current_build/
+// main.f90:170-171; file names package_library/src/global_data.f90:12-18):
+//   chemical_data.inf         &chemical_properties chemical_mechanism_file_name,
+//                             default_enhanced_efficiencies, E_act_units /
+//                             (chemical_properties_class.f90:77-97)
+//   thermophysical_setup.inf  &thermo_properties thermo_data_file_name,
+//                             transport_data_file_name, molar_masses_data_file_name /
+//                             (thermophysical_properties_class.f90:97-134)
+// and opens the named files under chemical_mechanisms/ and thermophysical_data/
+// of the same folder (chemical_properties_class.f90:135,
+// thermophysical_properties_class.f90:161-163).
+
+// One namelist group read from a file: lower-cased names -> value text
+// (character values unquoted, Fortran's doubled quotes undone, trailing blanks
+// dropped, as a character variable compares and as OPEN uses it).
+std::vector<std::pair<std::string, std::string>> read_namelist(const char *path, const std::string &group)
+{
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw FormatError{std::string("cannot open ") + path};
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string txt = ss.str();
+  auto lower = [](std::string v) {
+    for (char &c : v) c = (char)std::tolower((unsigned char)c);
+    return v;
+  };
+  // find "&group" (case-insensitive) outside comments
+  size_t i = 0;
+  const std::string want = lower(group);
+  bool found = false;
+  while (i < txt.size()) {
+    if (txt[i] == '!') {
+      while (i < txt.size() && txt[i] != '\n') ++i;
+      continue;
+    }
+    if (txt[i] == '&') {
+      size_t j = i + 1;
+      while (j < txt.size() && (std::isalnum((unsigned char)txt[j]) || txt[j] == '_')) ++j;
+      if (lower(txt.substr(i + 1, j - i - 1)) == want) {
+        i = j;
+        found = true;
+        break;
+      }
+    }
+    ++i;
+  }
+  if (!found) throw FormatError{std::string("no &") + group + " namelist group in " + path};
+  std::vector<std::pair<std::string, std::string>> items;
+  auto skip_sep = [&]() {
+    while (i < txt.size()) {
+      const char c = txt[i];
+      if (is_space(c) || c == ',') {
+        ++i;
+      } else if (c == '!') {
+        while (i < txt.size() && txt[i] != '\n') ++i;
+      } else {
+        break;
+      }
+    }
+  };
+  for (;;) {
+    skip_sep();
+    if (i >= txt.size()) throw FormatError{std::string("namelist &") + group + " not terminated in " + path};
+    if (txt[i] == '/') break;
+    if (txt[i] == '&') {                      // "&end"
+      size_t j = i + 1;
+      while (j < txt.size() && std::isalpha((unsigned char)txt[j])) ++j;
+      if (lower(txt.substr(i + 1, j - i - 1)) == "end") break;
+      throw FormatError{std::string("unexpected '&' in namelist &") + group + " of " + path};
+    }
+    size_t j = i;
+    while (j < txt.size() && (std::isalnum((unsigned char)txt[j]) || txt[j] == '_')) ++j;
+    if (j == i) throw FormatError{std::string("bad item in namelist &") + group + " of " + path};
+    const std::string name = lower(txt.substr(i, j - i));
+    i = j;
+    while (i < txt.size() && is_space(txt[i])) ++i;
+    if (i >= txt.size() || txt[i] != '=')
+      throw FormatError{"expected '=' after " + name + " in namelist &" + group + " of " + path};
+    ++i;
+    while (i < txt.size() && is_space(txt[i])) ++i;
+    std::string val;
+    if (i < txt.size() && (txt[i] == '\'' || txt[i] == '"')) {
+      const char q = txt[i++];
+      for (;;) {
+        if (i >= txt.size()) throw FormatError{"unterminated string for " + name + " in " + path};
+        if (txt[i] == q) {
+          if (i + 1 < txt.size() && txt[i + 1] == q) {
+            val += q;
+            i += 2;
+            continue;
+          }
+          ++i;
+          break;
+        }
+        val += txt[i++];
+      }
+      val = rstrip(val);
+    } else {
+      while (i < txt.size() && !is_space(txt[i]) && txt[i] != ',' && txt[i] != '/' && txt[i] != '!')
+        val += txt[i++];
+    }
+    items.emplace_back(name, val);
+  }
+  return items;
+}
+
+std::string namelist_value(const std::vector<std::pair<std::string, std::string>> &items,
+                           const std::string &name, const char *path)
+{
+  std::string v;
+  bool have = false;
+  for (const auto &kv : items)
+    if (kv.first == name) {            // a later assignment of the same name wins
+      v = kv.second;
+      have = true;
+    }
+  if (!have) throw FormatError{name + " is not set in " + path};
+  return v;
+}
+
 }  // namespace
 
 extern "C" {
+
+int nrg_mech_load_task_setup(const char *task_setup_dir, nrg_mech_file **out)
+{
+  if (!out) return set_error(NRG_CHEM_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (!task_setup_dir) return set_error(NRG_CHEM_E_ARG, "task set-up folder is NULL");
+  const std::string dir(task_setup_dir);
+  std::string mech, thermo, molar, units;
+  double eff = 0.0;
+  try {
+    const std::string chem_inf = dir + "/chemical_data.inf";
+    const std::string thermo_inf = dir + "/thermophysical_setup.inf";
+    const auto c = read_namelist(chem_inf.c_str(), "chemical_properties");
+    mech = namelist_value(c, "chemical_mechanism_file_name", chem_inf.c_str());
+    units = namelist_value(c, "e_act_units", chem_inf.c_str());
+    const std::string e = namelist_value(c, "default_enhanced_efficiencies", chem_inf.c_str());
+    if (!fortran_real(e, eff))
+      throw FormatError{"default_enhanced_efficiencies '" + e + "' is not a real in " + chem_inf};
+    const auto t = read_namelist(thermo_inf.c_str(), "thermo_properties");
+    thermo = namelist_value(t, "thermo_data_file_name", thermo_inf.c_str());
+    molar = namelist_value(t, "molar_masses_data_file_name", thermo_inf.c_str());
+  } catch (const FormatError &e) {
+    return set_error(NRG_CHEM_E_FORMAT, "%s", e.msg.c_str());
+  } catch (const std::exception &e) {
+    return set_error(NRG_CHEM_E_FORMAT, "%s", e.what());
+  }
+  const std::string mp = dir + "/chemical_mechanisms/" + mech;
+  const std::string tp = dir + "/thermophysical_data/" + thermo;
+  const std::string wp = dir + "/thermophysical_data/" + molar;
+  return nrg_mech_load(mp.c_str(), tp.c_str(), wp.c_str(), eff, units.c_str(), out);
+}
 
 int nrg_mech_load(const char *mechanism_path, const char *thermo_path, const char *molar_masses_path,
                   double default_enhanced_efficiencies, const char *E_act_units, nrg_mech_file **out)

@@ -558,6 +558,125 @@ def load_mechanism(mech_file="KEROMNES.txt", thermo_file="KEROMNES_THERMO.txt",
     return Mechanism(a_coeffs=a, molar_masses=mm, **d)
 
 
+def read_namelist(path, group):
+    """One namelist group of a Fortran namelist file, as {lower-case name: value}:
+    character values unquoted (doubled quotes undone, trailing blanks dropped),
+    other values as their text.  '!' starts a comment; '/' or '&end' ends the
+    group; a later assignment of a name wins."""
+    with open(path, "r", encoding="latin-1") as fh:
+        txt = fh.read()
+    txt = "\n".join(_strip_comment(line) for line in txt.split("\n"))
+    m = re.search(r"&" + re.escape(group) + r"\b", txt, flags=re.I)
+    if not m:
+        raise MechanismError("no &%s namelist group in %s" % (group, path))
+    i, out = m.end(), {}
+    n = len(txt)
+    while True:
+        while i < n and (txt[i].isspace() or txt[i] == ","):
+            i += 1
+        if i >= n:
+            raise MechanismError("namelist &%s not terminated in %s" % (group, path))
+        if txt[i] == "/":
+            break
+        if txt[i] == "&":
+            if txt[i + 1:i + 4].lower() == "end":
+                break
+            raise MechanismError("unexpected '&' in namelist &%s of %s" % (group, path))
+        m = re.match(r"[A-Za-z0-9_]+", txt[i:])
+        if not m:
+            raise MechanismError("bad item in namelist &%s of %s" % (group, path))
+        name = m.group(0).lower()
+        i += m.end()
+        while i < n and txt[i].isspace():
+            i += 1
+        if i >= n or txt[i] != "=":
+            raise MechanismError("expected '=' after %s in namelist &%s of %s" % (name, group, path))
+        i += 1
+        while i < n and txt[i].isspace():
+            i += 1
+        if i < n and txt[i] in "'\"":
+            q, i, val = txt[i], i + 1, []
+            while True:
+                if i >= n:
+                    raise MechanismError("unterminated string for %s in %s" % (name, path))
+                if txt[i] == q:
+                    if i + 1 < n and txt[i + 1] == q:
+                        val.append(q)
+                        i += 2
+                        continue
+                    i += 1
+                    break
+                val.append(txt[i])
+                i += 1
+            out[name] = "".join(val).rstrip()
+        else:
+            m = re.match(r"[^\s,/!]*", txt[i:])
+            out[name] = m.group(0)
+            i += m.end()
+    return out
+
+
+def _strip_comment(line):
+    """A namelist line without its '!' comment (quotes respected)."""
+    q = None
+    for k, c in enumerate(line):
+        if q:
+            if c == q:
+                q = None
+        elif c in "'\"":
+            q = c
+        elif c == "!":
+            return line[:k]
+    return line
+
+
+def task_setup_files(task_dir):
+    """The data files and settings NRG's main program reads from a task set-up
+    folder (computing_module/src/current_build/main.f90:170-171):
+    chemical_data.inf &chemical_properties
+    (package_library/src/chemical_properties_class.f90:77-97) and
+    thermophysical_setup.inf &thermo_properties
+    (package_library/src/thermophysical_properties_class.f90:97-134); the files
+    sit under chemical_mechanisms/ and thermophysical_data/
+    (chemical_properties_class.f90:135, thermophysical_properties_class.f90:161-163)."""
+    cp = os.path.join(task_dir, "chemical_data.inf")
+    tp = os.path.join(task_dir, "thermophysical_setup.inf")
+    c = read_namelist(cp, "chemical_properties")
+    t = read_namelist(tp, "thermo_properties")
+
+    def need(d, k, path):
+        if k not in d:
+            raise MechanismError("%s is not set in %s" % (k, path))
+        return d[k]
+
+    return dict(
+        mech_path=os.path.join(task_dir, "chemical_mechanisms", need(c, "chemical_mechanism_file_name", cp)),
+        thermo_path=os.path.join(task_dir, "thermophysical_data", need(t, "thermo_data_file_name", tp)),
+        molar_mass_path=os.path.join(task_dir, "thermophysical_data",
+                                     need(t, "molar_masses_data_file_name", tp)),
+        default_enhanced_efficiencies=_fortran_real(need(c, "default_enhanced_efficiencies", cp)),
+        E_act_units=need(c, "e_act_units", cp))
+
+
+def load_task_setup(task_dir):
+    """Mechanism + thermo of a task set-up folder, read as NRG's main program reads them."""
+    f = task_setup_files(task_dir)
+    d = parse_mechanism(f["mech_path"], f["default_enhanced_efficiencies"], f["E_act_units"])
+    a, mm = parse_thermo(d["species_names"], f["thermo_path"], f["molar_mass_path"])
+    return Mechanism(a_coeffs=a, molar_masses=mm, **d)
+
+
+def load_task_setup_native(task_dir):
+    """The same folder read by the library's C++ loader (nrg_mech_load_task_setup)."""
+    import ctypes as C
+
+    from . import capi
+    lib = capi.load_library()
+    h = C.c_void_p()
+    capi.check(lib, lib.nrg_mech_load_task_setup(str(task_dir).encode(), C.byref(h)))
+    return _native_mechanism(lib, h)
+
+
 def load_mechanism_native(mech_path, thermo_path, molar_mass_path,
                           default_enhanced_efficiencies=1.0, E_act_units="cal.mol"):
     """The same three files read by the library's C++ loader (include/nrg_mech.h,
@@ -571,6 +690,14 @@ def load_mechanism_native(mech_path, thermo_path, molar_mass_path,
                                       str(molar_mass_path).encode(),
                                       float(default_enhanced_efficiencies), E_act_units.encode(),
                                       C.byref(h)))
+    return _native_mechanism(lib, h)
+
+
+def _native_mechanism(lib, h):
+    """Copies of a loaded nrg_mech_file's arrays as a Mechanism; frees the handle."""
+    import ctypes as C
+
+    from . import capi
     try:
         v = capi.NrgChemMech()
         capi.check(lib, lib.nrg_mech_view(h, C.byref(v)))

@@ -183,3 +183,75 @@ def test_native_view_feeds_create():
         else:
             assert st == want
         lib.nrg_mech_free(h)
+
+
+def _task_setup(tmp, mech, thermo, chem_inf, thermo_inf):
+    """A task set-up folder with NRG's layout (package_library/src/global_data.f90:12-18)."""
+    import shutil
+    os.makedirs(os.path.join(tmp, "chemical_mechanisms"))
+    os.makedirs(os.path.join(tmp, "thermophysical_data"))
+    shutil.copy(mech, os.path.join(tmp, "chemical_mechanisms"))
+    shutil.copy(thermo, os.path.join(tmp, "thermophysical_data"))
+    shutil.copy(MM, os.path.join(tmp, "thermophysical_data"))
+    with open(os.path.join(tmp, "chemical_data.inf"), "w") as fh:
+        fh.write(chem_inf)
+    with open(os.path.join(tmp, "thermophysical_setup.inf"), "w") as fh:
+        fh.write(thermo_inf)
+    return str(tmp)
+
+
+# as Fortran's namelist WRITE with delim='quote' leaves them: upper-case names,
+# values padded to the variable's length (character(len=50) / len=15 / len=30)
+FORTRAN_CHEM = ('&CHEMICAL_PROPERTIES\n CHEMICAL_MECHANISM_FILE_NAME="%-50s",\n'
+                ' DEFAULT_ENHANCED_EFFICIENCIES=  %s     ,\n E_ACT_UNITS="%-15s"\n /\n')
+FORTRAN_THERMO = ('&THERMO_PROPERTIES\n THERMO_DATA_FILE_NAME="%-30s",\n'
+                  ' TRANSPORT_DATA_FILE_NAME="%-30s",\n MOLAR_MASSES_DATA_FILE_NAME="%-30s"\n /\n')
+
+
+@pytest.mark.parametrize("name,eff,units", [("ZHANG", "1.0000000000000000", "cal.mol"),
+                                            ("KEROMNES", "0.50000000000000000", "kJ.mol")])
+def test_task_setup_as_fortran_writes_it(tmp_path, name, eff, units):
+    """chemical_data.inf / thermophysical_setup.inf written as NRG's write_properties
+    writes them: both loaders read the named files with the namelist's settings,
+    bit for bit the arrays of the direct load."""
+    mech, thermo = CASES[name]
+    d = _task_setup(tmp_path, mech, thermo,
+                    FORTRAN_CHEM % (os.path.basename(mech), eff, units),
+                    FORTRAN_THERMO % (os.path.basename(thermo), "transport.dat", "molar_masses.dat"))
+    direct = _python(mech, thermo, default_enhanced_efficiencies=float(eff), E_act_units=units)
+    _same(direct, M.load_task_setup(d))
+    _same(direct, M.load_task_setup_native(d))
+    f = M.task_setup_files(d)
+    assert f["E_act_units"] == units and f["default_enhanced_efficiencies"] == float(eff)
+
+
+def test_task_setup_namelist_forms(tmp_path):
+    """Hand-written namelists: lower case, apostrophes, comments, one line,
+    '&end', a reassigned name (the later one wins)."""
+    mech, thermo = CASES["ZHANG"]
+    chem = ("! chemistry of the task\n&chemical_properties  e_act_units = 'cal.mol',"
+            " chemical_mechanism_file_name='KEROMNES.txt' ! overridden below\n"
+            "  default_enhanced_efficiencies=1.0 chemical_mechanism_file_name = 'ZHANG.txt'\n&end\n")
+    therm = "&Thermo_Properties thermo_data_file_name='ZHANG_THERMO.txt', " \
+            "molar_masses_data_file_name='molar_masses.dat' /\n"
+    d = _task_setup(tmp_path, mech, thermo, chem, therm)
+    direct = _python(mech, thermo)
+    _same(direct, M.load_task_setup(d))
+    _same(direct, M.load_task_setup_native(d))
+
+
+def test_task_setup_errors(tmp_path):
+    mech, thermo = CASES["ZHANG"]
+    d = _task_setup(tmp_path, mech, thermo,
+                    "&chemical_properties chemical_mechanism_file_name='ZHANG.txt', "
+                    "default_enhanced_efficiencies=1.0 /\n",
+                    "&thermo_properties thermo_data_file_name='ZHANG_THERMO.txt', "
+                    "molar_masses_data_file_name='molar_masses.dat' /\n")
+    with pytest.raises(M.MechanismError):
+        M.load_task_setup(d)                     # E_act_units missing
+    lib = capi.load_library()
+    h = C.c_void_p()
+    assert lib.nrg_mech_load_task_setup(d.encode(), C.byref(h)) == capi.NRG_CHEM_E_FORMAT
+    assert b"e_act_units" in lib.nrg_chem_last_error()
+    assert lib.nrg_mech_load_task_setup(str(tmp_path / "nowhere").encode(),
+                                        C.byref(h)) == capi.NRG_CHEM_E_FORMAT
