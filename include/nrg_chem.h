@@ -218,6 +218,30 @@ int nrg_chem_step_dev_diag(nrg_chem_t *h, const nrg_grid *grid, double dt,
                            int32_t *cell_status, nrg_chem_stats *stats,
                            int32_t *nstep, int32_t *nfe, int32_t *nje);
 
+/*
+ * nrg_chem_step_dev that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot call hipMalloc (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  Same arguments,
+ * kernels, arithmetic and outputs as nrg_chem_step_dev.  The work counters,
+ * queue heads and the rate constants of one chunk live in a fixed device
+ * array of NRG_NOALLOC_SCRATCH_BYTES that the library holds on every device
+ * it runs on until the process ends; the grid is integrated in chunks of
+ * NRG_NOALLOC_CHUNK_CELLS cells in natural cell order (the handle's cell_order
+ * and use_graph are not applied; results are the same).  A grid of more than
+ * NRG_NOALLOC_MAX_CELLS inner cells returns NRG_CHEM_E_ARG before any work.
+ * All handles on one device share that array and each call leaves its
+ * contents behind: calls that overlap on one device (other host threads,
+ * other handles' streams) race and must be serialised by the caller.
+ */
+#define NRG_NOALLOC_SCRATCH_BYTES (64u << 20)          /* 64 MiB per device             */
+#define NRG_NOALLOC_CHUNK_CELLS (1 << 17)              /* cells per integrator launch   */
+#define NRG_NOALLOC_MAX_CELLS (48 * NRG_NOALLOC_CHUNK_CELLS)  /* 6,291,456 inner cells  */
+int nrg_chem_step_dev_noalloc(nrg_chem_t *h, const nrg_grid *grid, double dt,
+                              const double *T, const double *p, const double *rho,
+                              const double *const *Y, const int8_t *bc,
+                              double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                              int32_t *cell_status, nrg_chem_stats *stats);
+
 /* Same contract with host pointers (fields are staged through device buffers). */
 int nrg_chem_step(nrg_chem_t *h, const nrg_grid *grid, double dt,
                   const double *T, const double *p, const double *rho,

@@ -40,6 +40,10 @@ NRG_FORM_PERSISTENT = 1
 NRG_ORDER_NATURAL = 0
 NRG_ORDER_STIFFNESS = 1
 
+NRG_NOALLOC_SCRATCH_BYTES = 64 << 20    # nrg_chem_step_dev_noalloc
+NRG_NOALLOC_CHUNK_CELLS = 1 << 17
+NRG_NOALLOC_MAX_CELLS = 48 * NRG_NOALLOC_CHUNK_CELLS
+
 NRG_CELL_SURROGATE = 6          # include/nrg_surrogate.h
 NRG_SUR_MAX_IN = 16
 NRG_SUR_MAX_OUT = 16
@@ -311,6 +315,8 @@ def load_library(path=None):
     lib.nrg_chem_step.restype = C.c_int
     lib.nrg_chem_step_dev_diag.argtypes = step_args + [vp, vp, vp]
     lib.nrg_chem_step_dev_diag.restype = C.c_int
+    lib.nrg_chem_step_dev_noalloc.argtypes = step_args
+    lib.nrg_chem_step_dev_noalloc.restype = C.c_int
     lib.nrg_chem_closure_dev.argtypes = [vp, C.POINTER(NrgGrid), C.c_double, vp, vp, C.c_int32,
                                          vp, vp, vp, vp, vp, vp, vp]
     lib.nrg_chem_closure_dev.restype = C.c_int

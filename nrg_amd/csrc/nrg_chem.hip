@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1272,6 +1273,21 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 }  // namespace
 
 // ===========================================================================
+// Scratch of nrg_chem_step_dev_noalloc: the work accumulators and queue heads
+// (ACC_SLOTS words, padded to 512 bytes), then the rate constants of one chunk
+// of NRG_NOALLOC_CHUNK_CELLS cells.  One copy per device, held until the
+// process ends.  Host code reaches it only through the per-device address
+// hipGetSymbolAddress returned (g_noalloc_scratch), never through its name.
+// ===========================================================================
+constexpr size_t NOALLOC_ACC_BYTES = 512;
+static_assert(ACC_SLOTS * sizeof(unsigned long long) <= NOALLOC_ACC_BYTES, "accumulators");
+static_assert(NOALLOC_ACC_BYTES + (size_t)NRG_NOALLOC_CHUNK_CELLS * (2 * R + NM) * sizeof(double) <=
+                  (size_t)NRG_NOALLOC_SCRATCH_BYTES, "rate constants of one chunk");
+static_assert((long long)NRG_NOALLOC_MAX_CELLS <= (long long)MAX_CHUNKS * NRG_NOALLOC_CHUNK_CELLS,
+              "one queue head per chunk");
+static __device__ __align__(256) unsigned char scratch[NRG_NOALLOC_SCRATCH_BYTES];
+
+// ===========================================================================
 // Host side
 // ===========================================================================
 // What a captured step graph depends on: its kernel arguments and the queue set-up.
@@ -1496,6 +1512,28 @@ int check_device(int dev)
 
 }  // namespace
 
+// Device address of `scratch` on each device, taken once per device (with that
+// device current) when the first handle on it is created.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void *g_noalloc_scratch[NOALLOC_MAX_DEVICES];
+static std::mutex g_noalloc_mu;
+
+static int noalloc_setup(int dev)
+{
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return set_error(NRG_CHEM_E_NO_DEVICE, "device %d beyond the %d the library serves", dev,
+                     NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_scratch[dev]) return NRG_CHEM_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_error(NRG_CHEM_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                     hipGetErrorString(e));
+  g_noalloc_scratch[dev] = ptr;
+  return NRG_CHEM_OK;
+}
+
 // Blocks of 256 threads covering the aligned walk, one offset per thread.
 static long long walk_blocks(const GridIdx &g) { return std::max<long long>(1, (g.pad1 - g.pad0 + 255) / 256); }
 
@@ -1597,6 +1635,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
   };
   if (hipSetDevice(h->device) != hipSuccess) return fail(set_error(NRG_CHEM_E_HIP, "hipSetDevice"));
   if (hipStreamCreate(&h->stream) != hipSuccess) return fail(set_error(NRG_CHEM_E_HIP, "hipStreamCreate"));
+  if ((st = noalloc_setup(h->device))) return fail(st);
   if (hipMalloc(&h->dev_mech, sizeof(MechCoef)) != hipSuccess)
     return fail(set_error(NRG_CHEM_E_HIP, "hipMalloc mech"));
   if (hipMalloc(&h->dev_acc, ACC_SLOTS * sizeof(unsigned long long)) != hipSuccess)
@@ -1719,18 +1758,18 @@ static int prepare_rates(nrg_chem_t *h, long long npos)
   return NRG_CHEM_OK;
 }
 
-static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
+// Launch rates_kernel + chem_bdf_kernel per chunk of `chunk` positions, with
+// the chunk's rate constants in `rates` and its queue head in acc[ACC_QUEUE0 + ci].
+static void enqueue_chunks(nrg_chem_t *h, StepArgs &a, long long npos, long long chunk,
+                           double *rates, unsigned long long *acc)
 {
-  int st = prepare_rates(h, npos);
-  if (st) return st;
-  const long long chunk = chunk_for(h, npos);
   const long long nchunks = npos > 0 ? (npos + chunk - 1) / chunk : 0;
-  a.rates = h->rates;
-  a.rates_out = h->rates;
+  a.rates = rates;
+  a.rates_out = rates;
   for (long long ci = 0; ci < nchunks; ++ci) {
     a.base = ci * chunk;
     a.nchunk = std::min<long long>(chunk, npos - a.base);
-    a.queue = h->dev_acc + ACC_QUEUE0 + ci;
+    a.queue = acc + ACC_QUEUE0 + ci;
     hipLaunchKernelGGL(rates_kernel, dim3((unsigned)((a.nchunk + RATES_CELLS - 1) / RATES_CELLS)),
                        dim3(256), 0, h->stream, a);
     const long long need = (a.nchunk + WAVE - 1) / WAVE;
@@ -1748,6 +1787,13 @@ static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
         hipLaunchKernelGGL((chem_bdf_kernel<false, false>), dim3((unsigned)need), dim3(WAVE), 0, h->stream, a);
     }
   }
+}
+
+static int enqueue_integrator(nrg_chem_t *h, StepArgs &a, long long npos)
+{
+  int st = prepare_rates(h, npos);
+  if (st) return st;
+  enqueue_chunks(h, a, npos, chunk_for(h, npos), h->rates, h->dev_acc);
   HIP_TRY(hipGetLastError());
   return NRG_CHEM_OK;
 }
@@ -1829,17 +1875,17 @@ static int launch_step_graph(nrg_chem_t *h, StepArgs &a, bool use_hist)
   return NRG_CHEM_OK;
 }
 
-static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
-                         const double *rho, const double *const *Y, const int8_t *bc,
-                         double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
-                         int32_t *cell_status, nrg_chem_stats *stats, int32_t *nstep,
-                         int32_t *nfe, int32_t *nje)
+// Step arguments from an entry point's pointers (checked); the work counters,
+// accumulators and queue are left for the caller to set.
+static int fill_step_args(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                          const double *rho, const double *const *Y, const int8_t *bc,
+                          double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                          int32_t *cell_status, StepArgs &a)
 {
   if (!h) return set_error(NRG_CHEM_E_ARG, "handle is NULL");
   if (!(dt > 0.0) || !std::isfinite(dt)) return set_error(NRG_CHEM_E_ARG, "time_step must be > 0");
   if (!T || !rho || !Y || !Y_prod || !Y_prod2 || !E_f_prod)
     return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL");
-  StepArgs a{};
   int st = make_grid(grid, a.g);
   if (st) return st;
   for (int s = 0; s < S; ++s) {
@@ -1856,11 +1902,23 @@ static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const d
   a.bc = bc;
   a.Efp = E_f_prod;
   a.status = cell_status;
+  a.jac_pct = h->jac_pct;
+  return NRG_CHEM_OK;
+}
+
+static int step_dev_impl(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                         const double *rho, const double *const *Y, const int8_t *bc,
+                         double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
+                         int32_t *cell_status, nrg_chem_stats *stats, int32_t *nstep,
+                         int32_t *nfe, int32_t *nje)
+{
+  StepArgs a{};
+  int st = fill_step_args(h, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status, a);
+  if (st) return st;
   a.nstep = nstep;
   a.nfe = nfe;
   a.nje = nje;
   a.acc = h->dev_acc;
-  a.jac_pct = h->jac_pct;
   HIP_TRY(hipSetDevice(h->device));
   unsigned long long acc[16] = {};
   float ms = 0.f;
@@ -1918,6 +1976,44 @@ int nrg_chem_step_dev_diag(nrg_chem_t *h, const nrg_grid *grid, double dt, const
     return set_error(NRG_CHEM_E_ARG, "nstep, nfe and nje must be all NULL or all set");
   return step_dev_impl(h, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status, stats,
                        nstep, nfe, nje);
+}
+
+// nrg_chem_step_dev with every device buffer beyond the caller's fields taken
+// from `scratch` (through its per-device address): no allocation, no graph,
+// natural cell order, chunks of NRG_NOALLOC_CHUNK_CELLS cells.  The kernels and
+// their arithmetic are those of nrg_chem_step_dev.
+int nrg_chem_step_dev_noalloc(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                              const double *p, const double *rho, const double *const *Y,
+                              const int8_t *bc, double *const *Y_prod, double *const *Y_prod2,
+                              double *E_f_prod, int32_t *cell_status, nrg_chem_stats *stats)
+{
+  (void)p;
+  StepArgs a{};
+  int st = fill_step_args(h, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status, a);
+  if (st) return st;
+  if (a.g.ninner > (long long)NRG_NOALLOC_MAX_CELLS)
+    return set_error(NRG_CHEM_E_ARG,
+                     "%lld inner cells: nrg_chem_step_dev_noalloc serves at most %lld "
+                     "(NRG_NOALLOC_MAX_CELLS); use nrg_chem_step_dev",
+                     a.g.ninner, (long long)NRG_NOALLOC_MAX_CELLS);
+  void *ptr = (h->device >= 0 && h->device < NOALLOC_MAX_DEVICES) ? g_noalloc_scratch[h->device] : nullptr;
+  if (!ptr) return set_error(NRG_CHEM_E_HIP, "no scratch address for device %d", h->device);
+  auto *acc_dev = (unsigned long long *)ptr;
+  auto *rates = (double *)((unsigned char *)ptr + NOALLOC_ACC_BYTES);
+  a.acc = acc_dev;
+  HIP_TRY(hipSetDevice(h->device));
+  unsigned long long acc[16] = {};
+  float ms = 0.f;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, acc_dev, ACC_SLOTS);
+  if (a.g.ninner > 0) enqueue_chunks(h, a, a.g.ninner, NRG_NOALLOC_CHUNK_CELLS, rates, acc_dev);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipMemcpyAsync(acc, acc_dev, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (stats) fill_stats(acc, a.g.ninner, ms, stats);
+  return integrator_status(acc);
 }
 
 // Host fields: stage them through the handle's device buffers, run the FP64
