@@ -35,9 +35,12 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="cells per dimension")
     ap.add_argument("--dt", type=float, default=1.0e-7)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-sample", type=int, default=16384,
-                    help="cells of the workload timed on the host through the oracle (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 20,
+                    help="cells of the initial state timed on the host cores (the reference's "
+                         "SLATEC integrator, and the oracle) and audited (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: every core this process may run on, at most 16 (one GPU's share "
+                         "of the host)")
     ap.add_argument("--form", choices=["plain", "persistent"], default="persistent",
                     help="integrator launch form (same results)")
     ap.add_argument("--jacobian", choices=["fd", "analytic"], default="fd",
@@ -65,10 +68,20 @@ def parse():
     ap.add_argument("--gate-dy-freeze", type=float, default=None)
     ap.add_argument("--gate-rate-tol", type=float, default=None,
                     help="mass fraction; 0 turns the rate-consistency test off")
+    ap.add_argument("--screen", choices=["none", "rates"], default="rates",
+                    help="cells the network sees: 'rates': those the explicit-rate screen does "
+                         "not freeze; 'none': every live cell")
+    ap.add_argument("--no-network", type=int, default=1,
+                    help="also time the method without the network (explicit-rate screen, FP64 "
+                         "for every cell it does not freeze) from the same initial state")
     ap.add_argument("--table", type=int, default=1,
                     help="time the table_approximated mode beside the FP64 integrator (0: skip)")
     ap.add_argument("--low-mach", type=int, default=1,
                     help="time the low-Mach coupling kernels on the FP64 run's outputs (0: skip)")
+    ap.add_argument("--uneven", type=int, default=1,
+                    help="time the uneven-stiffness 256^3 field (5%% of cells in the 1000-1300 K "
+                         "ignition window) at dt 1e-8, 1e-7, 1e-6 s: persistent graph queue vs "
+                         "one cell per lane (0: skip)")
     ap.add_argument("--table-line", type=int, default=512,
                     help="cells of the flame line the table is written from")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -208,8 +221,9 @@ def main():
     tr = profiled_traffic(ncells)
     roofline = {
         "kernel": "rates_kernel + chem_bdf_kernel (one chemistry step)",
-        "bound": "mfma",
-        "compute": "fp64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+        "bound": "fp64_valu",
+        "compute": "FP64 vector ALU (the 9x9 LU and mass-action work is per-lane vector code; "
+                   "MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
         "achieved": round(achieved_tflops, 4),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
@@ -220,30 +234,19 @@ def main():
         "dispatches_per_call": nchunks,
         "algorithmic_flops_per_launch": mean_flops,
         "hbm_boundary_GBs": round(flop_model.BYTES_BOUNDARY * ncells / (mean_ms * 1e-3) / 1e9, 2),
+        "lane_utilisation": profiled_lanes(),
+        "lane_utilisation_source": os.path.relpath(SQ_PROFILE, ROOT),
+        "chem_bdf_kernel_resources": kernel_resources(
+            "chem_bdf_kernelILb1ELb0E" if effective_setup(args, tuning)["form"] == "persistent"
+            else "chem_bdf_kernelILb0ELb0E"),
     }
 
-    # ---- CPU baseline and audit on the sample (oracle = CPU restatement) ------------------
+    # ---- CPU baseline and audit on the sample ---------------------------------------------
     cpu_baseline = None
     audit = None
     if sample is not None:
-        from oracle import Oracle
-        orc = Oracle(mech)
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu_baseline, o, (E2, Y2, T2) = cpu_leg(args, mech, sample)
         m = len(sample["idx"])
-        c0 = time.perf_counter()
-        o = orc.chem_cells(args.dt, sample["T"], sample["rho"], sample["Y"], nthreads=threads)
-        E2, Y2, T2, _ = orc.closure_cells(args.dt, sample["rho"], o["E_f_prod"], o["Y_prod"],
-                                          sample["E_f"], sample["Y"], sample["T"],
-                                          nthreads=threads)
-        c1 = time.perf_counter()
-        cpu_baseline = {
-            "value": round(m / (c1 - c0), 1),
-            "unit": "cell-steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": "%d uniformly sampled cells of the initial 256^3 state, one split step "
-                      "(chemistry + closure), C oracle with OpenMP" % m,
-        }
         if audit_out is not None:
             dY = np.abs(audit_out["Y"] - Y2).max()
             relT = float((np.abs(audit_out["T"] - T2) / T2).max())
@@ -261,13 +264,28 @@ def main():
         path = os.path.join(ROOT, "nrg_amd", "data", "surrogate_keromnes.npz")
         path = path if os.path.exists(path) else "none"
     if path != "none":
-        sur = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S)
+        sur = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S,
+                            screen=args.screen)
+        if args.no_network:
+            base = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S,
+                                 screen="only")
+            sur["no_network"] = base
+            sur["network_gain"] = round(sur["value"] / base["value"], 4)
+        sur["fallback_nfe_share_of_fp64"] = round(sur["fallback_nfe_per_step"] /
+                                                  (nfe / args.steps), 4)
 
     # ---- low-Mach (FDS) coupling kernels on the FP64 run's outputs ------------------------
     lm = low_mach_leg(args, dm, solver, S) if args.low_mach else None
 
     # ---- table-approximated mode from the same initial state -----------------------------
     tab = table_leg(args, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S) if args.table else None
+
+    # ---- uneven stiffness: persistent graph queue vs one cell per lane --------------------
+    uneven = None
+    if args.uneven:
+        del fields0, Y0, T_ref, Y_ref
+        torch.cuda.empty_cache()
+        uneven = uneven_leg(args, mech, tuning)
 
     line = {
         "metric": METRIC,
@@ -297,8 +315,73 @@ def main():
         "surrogate": sur,
         "table_mode": tab,
         "low_mach": lm,
+        "uneven_stiffness": uneven,
     }
     print(json.dumps(line), flush=True)
+
+
+def host_cpu():
+    """Model name and core counts of the host this process runs on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count() or 1, "usable": usable}
+
+
+def cpu_leg(args, mech, sample):
+    """One split step of the sampled cells on the host cores, timed two ways:
+    'reference': the reference's own SLATEC DDRIV3 (its Fortran sources built
+    with flang, oracle/_ref) integrating every cell, with the oracle's rate
+    constants, right-hand side and closure around it; 'port': the oracle's C
+    restatement of the whole step.  Both give the same numbers bit for bit.
+    Returns the baseline record, the chemistry outputs and the closed state."""
+    import numpy as np
+
+    from oracle import Oracle
+    from oracle.pyoracle import REF_SO, RefSlatec
+    orc = Oracle(mech)
+    cpu = host_cpu()
+    threads = args.cpu_threads or min(16, cpu["usable"])
+    m = len(sample["idx"])
+
+    def timed(chem):
+        c0 = time.perf_counter()
+        o = chem()
+        closed = orc.closure_cells(args.dt, sample["rho"], o["E_f_prod"], o["Y_prod"],
+                                   sample["E_f"], sample["Y"], sample["T"], nthreads=threads)
+        return o, closed[:3], time.perf_counter() - c0
+
+    o_port, closed, t_port = timed(lambda: orc.chem_cells(args.dt, sample["T"], sample["rho"],
+                                                          sample["Y"], nthreads=threads))
+    rec = {"unit": "cell-steps/s", "cores": threads, "cpu_model": cpu["model"],
+           "nproc": cpu["nproc"], "usable_cores": cpu["usable"],
+           "port_value": round(m / t_port, 1)}
+    if os.path.exists(REF_SO):
+        ref = RefSlatec()
+        o_ref, closed_ref, t_ref = timed(lambda: ref.chem_cells(orc, args.dt, sample["T"],
+                                                                sample["rho"], sample["Y"],
+                                                                nthreads=threads))
+        same = all(np.array_equal(o_ref[k], o_port[k]) for k in ("Y_prod", "E_f_prod", "nfe"))
+        rec.update(value=round(m / t_ref, 1), kind="reference", same_as_port=bool(same),
+                   sample="%d uniformly sampled cells of the initial state, one split step on %d "
+                          "threads: the reference's SLATEC DDRIV3 (flang build of its sources) "
+                          "per cell, with the C restatement's rate constants, right-hand side "
+                          "and closure" % (m, threads))
+    else:
+        rec.update(value=rec["port_value"], kind="port",
+                   sample="%d uniformly sampled cells of the initial state, one split step "
+                          "(chemistry + closure), C restatement on %d threads" % (m, threads))
+    rec["cpu_seconds"] = round((t_ref if rec["kind"] == "reference" else t_port) * threads, 1)
+    return rec, o_port, closed
 
 
 PEAK_BF16_TFLOPS = 2500.0
@@ -306,6 +389,41 @@ PEAK_HBM_GBS = 8000.0
 TABLE_BYTES_INTEGRATED = 8 + 1 + 9 * 8 + (9 + 9) * 8 + 8 + 4
 TABLE_BYTES_GATED = 8 + 1 + 9 * 8 + 8 + 4
 HBM_PROFILE = os.path.join(ROOT, "profiles", "chem_step_256cube_hbm_bytes.txt")
+SQ_PROFILE = os.path.join(ROOT, "profiles", "chem_step_256cube_sq_counters.txt")
+RESOURCE_USAGE = os.path.join(ROOT, "nrg_amd", "csrc", "resource_usage.txt")
+
+
+def profiled_lanes(kernel="chem_bdf_kernel"):
+    """Share of a wave's 64 lanes busy per VALU cycle of `kernel`, from the
+    committed SQ counter summary (LANES lines), or None."""
+    if not os.path.exists(SQ_PROFILE):
+        return None
+    for line in open(SQ_PROFILE):
+        if line.startswith("LANES"):
+            _, k, v = line.rstrip("\n").split("\t")
+            if k == kernel:
+                return float(v)
+    return None
+
+
+def kernel_resources(name="chem_bdf_kernelILb1ELb0E"):
+    """VGPRs, AGPRs, scratch bytes per lane and occupancy of a kernel, from the
+    compiler's resource report written when the library was built, or None."""
+    if not os.path.exists(RESOURCE_USAGE):
+        return None
+    out, cur = {}, False
+    for line in open(RESOURCE_USAGE):
+        if "Function Name:" in line:
+            cur = name in line
+            continue
+        if cur:
+            m = re.search(r"(VGPRs|AGPRs|ScratchSize \[bytes/lane\]|Occupancy \[waves/SIMD\]): (\d+)",
+                          line)
+            if m:
+                key = {"VGPRs": "vgprs", "AGPRs": "agprs", "ScratchSize [bytes/lane]":
+                       "scratch_bytes_per_lane", "Occupancy [waves/SIMD]": "waves_per_simd"}[m.group(1)]
+                out[key] = int(m.group(2))
+    return out or None
 
 
 def profiled_traffic(cells_per_call, kernels=("chem_bdf_kernel", "rates_kernel")):
@@ -430,6 +548,70 @@ def low_mach_leg(args, dm, solver, S, reps=10):
             "note": "median of %d calls, HIP events on the library's stream" % reps}
 
 
+def uneven_leg(args, mech, tuning, dts=(1e-8, 1e-7, 1e-6), reps=2):
+    """The uneven-stiffness workload: n^3 cells, 5 % of them fresh mixture in the
+    1000-1300 K ignition window (tens to hundreds of steps each), the rest burnt
+    or cold (a few steps).  For each dt, the chemistry step (rates_kernel +
+    chem_bdf_kernel, device time from HIP events) in two launch forms from the
+    same state: the persistent work queue in a HIP graph with stiffness order and
+    the main leg's autotuned queue set-up, and the plain one-cell-per-lane launch
+    in natural order.  Both give the same outputs bit for bit (checked here)."""
+    import numpy as np
+    import torch
+
+    from nrg_amd import capi, flop_model, inputs
+    from nrg_amd.solver import ChemistryStep
+    S = mech.species_number
+    n = args.n ** 3
+    st = inputs.ignition_mix(mech, n, seed=args.seed, device="cuda:0", frac_window=0.05)
+    T, rho = st["T"], st["rho"]
+    Y = st["Y"].contiguous()
+    grid = capi.flat_grid(n)
+    outs = [[torch.zeros(n, dtype=torch.float64, device="cuda:0") for _ in range(2 * S + 1)]
+            for _ in range(2)]
+    b = tuning["best"] if tuning else {}
+    queued = capi.default_opts(form=capi.NRG_FORM_PERSISTENT, cell_order=capi.NRG_ORDER_STIFFNESS,
+                               use_graph=1, jac_batch_pct=b.get("jac_batch_pct", 0),
+                               waves_per_cu=b.get("waves_per_cu", 0),
+                               chunk_log2=(int(b["chunk_cells"]).bit_length() - 1) if b else 0,
+                               errtest_gather_pct=b.get("errtest_gather_pct", 0))
+    plain = capi.default_opts(form=capi.NRG_FORM_PLAIN, cell_order=capi.NRG_ORDER_NATURAL,
+                              use_graph=0, chunk_log2=(n - 1).bit_length() if n > 1 else 16)
+    res = {"cells": n, "window_frac": 0.05, "reps": reps, "per_dt": []}
+    engines = {"persistent_graph_queue": ChemistryStep(mech, queued),
+               "one_cell_per_lane": ChemistryStep(mech, plain)}
+    try:
+        for dt in dts:
+            row = {"dt": dt}
+            for slot, (name, eng) in enumerate(engines.items()):
+                o = outs[slot]
+                call = lambda: eng.step_dev(  # noqa: E731
+                    grid, dt, T.data_ptr(), rho.data_ptr(), [Y[s].data_ptr() for s in range(S)],
+                    [o[s].data_ptr() for s in range(S)], [o[S + s].data_ptr() for s in range(S)],
+                    o[2 * S].data_ptr())
+                call()                                   # warm-up (and step-count history)
+                recs = [call() for _ in range(reps)]
+                ms = float(np.mean([r.kernel_ms for r in recs]))
+                r = recs[-1]
+                fl = flop_model.step_flops(r)
+                row[name] = {
+                    "kernel_ms": round(ms, 3),
+                    "cell_steps_per_s": round(n / (ms * 1e-3), 1),
+                    "fp64_tflops": round(fl / (ms * 1e-3) / 1e12, 4),
+                    "frac_fp64_peak": round(fl / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 5),
+                    "integrated_cells": int(r.cells_integrated),
+                    "nfe_per_integrated_cell": round(r.nfe_sum / max(1, r.cells_integrated), 2),
+                    "nstep_max": int(r.nstep_max)}
+            row["queue_speedup"] = round(row["one_cell_per_lane"]["kernel_ms"] /
+                                         row["persistent_graph_queue"]["kernel_ms"], 3)
+            row["identical_outputs"] = bool(all(torch.equal(a, c) for a, c in zip(*outs)))
+            res["per_dt"].append(row)
+    finally:
+        for eng in engines.values():
+            eng.close()
+    return res
+
+
 def table_leg(args, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S):
     """NRG's table_approximated mode as a non-learned baseline: a table dC(T)
     written from this library's FP64 step on a 1-D 15 % H2-air flame line at
@@ -489,17 +671,22 @@ def table_leg(args, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S):
     }
 
 
-def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S):
-    """Time the learned split step (network + gate + FP64 fallback + closure)
-    over the same number of steps from the same initial state, and audit its
-    final state against the FP64 integrator's."""
+def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S,
+                  screen="rates"):
+    """Time the learned split step (explicit-rate screen, network, gate, FP64
+    fallback, closure) over the same number of steps from the same initial
+    state, and audit its final state against the FP64 integrator's.  With
+    screen='only' the network is not used: every cell the explicit-rate screen
+    does not freeze goes to the FP64 integrator (the method without the
+    network)."""
     import numpy as np
     import torch
 
+    from nrg_amd import capi
     from nrg_amd import surrogate as sg
     restore_state(dm, fields0, Y0, S)
     z = np.load(path, allow_pickle=False)
-    mlp, norm = sg.load(path)
+    mlp, norm = sg.load(path) if screen != "only" else (None, None)
     gkw = {}
     if args.gate_energy_tol is not None:
         gkw["energy_tol"] = args.gate_energy_tol
@@ -514,7 +701,9 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_
     for k in ("dT_max", "dT_freeze", "dY_freeze", "energy_tol", "elem_tol", "elem_floor", "rate_tol"):
         if k not in gkw and ("gate_" + k) in z.files:
             gkw[k] = float(z["gate_" + k])
-    gate = sg.gate_struct(mech, **gkw)
+    scr = {"none": capi.NRG_SUR_SCREEN_NONE, "rates": capi.NRG_SUR_SCREEN_RATES,
+           "only": capi.NRG_SUR_SCREEN_ONLY}[screen]
+    gate = sg.gate_struct(mech, screen=scr, **gkw)
     chem = sg.SurrogateChemistry(solver, mlp, norm, gate)
 
     def step():
@@ -535,14 +724,18 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_
     value = ncells * args.steps / wall
     mean = lambda f: float(np.mean([f(r) for r in recs]))
     infer_ms = mean(lambda r: r.infer_ms)
-    flops = 2 * (12 * 256 + 3 * 256 * 256 + 256 * 10) * ncells      # 404,480 per inferred cell
-    tflops = flops / (infer_ms * 1e-3) / 1e12
+    net_cells = mean(lambda r: r.cells_network)
+    flops = 2 * (12 * 256 + 3 * 256 * 256 + 256 * 10) * net_cells   # 404,480 per inferred cell
+    tflops = flops / (infer_ms * 1e-3) / 1e12 if infer_ms > 0 else 0.0
     res = {
         "value": round(value, 1),
         "unit": "cell-steps/s",
         "ms_per_step": round(wall / args.steps * 1e3, 3),
         "speedup_vs_fp64": round(value / fp64_value, 3),
-        "dtype": "bf16 (fp32 accumulate) network, f64 gate and fallback",
+        "screen": screen,
+        "dtype": ("f64 screen and fallback (no network)" if screen == "only" else
+                  "bf16 (fp32 accumulate) network, f64 screen, gate and fallback"),
+        "network_frac": round(net_cells / ncells, 6),
         "fallback_frac": round(mean(lambda r: r.cells_fallback) / ncells, 6),
         "accepted_frac": round(mean(lambda r: r.cells_accepted) / ncells, 6),
         "frozen_frac": round(mean(lambda r: r.cells_frozen) / ncells, 6),
@@ -552,15 +745,19 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_
                              ("reject_domain", "reject_confidence", "reject_elements",
                               "reject_negative", "reject_energy", "reject_rate")},
         "gate": gkw,
+        "screen_ms": round(mean(lambda r: r.screen_ms), 3),
         "infer_ms": round(infer_ms, 3), "gate_ms": round(mean(lambda r: r.gate_ms), 3),
         "fallback_ms": round(mean(lambda r: r.fallback_ms), 3),
-        "roofline_infer": {"bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_BF16_TFLOPS,
-                           "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 4),
-                           "kernel": "featurize_kernel + mlp_infer_kernel (404,480 FLOP/cell)"},
         "audit_vs_fp64": audit_against(dm, T_ref, Y_ref, S, args.warmup + args.steps),
-        "model": os.path.relpath(path, ROOT),
     }
-    mlp.close()
+    if mlp is not None:
+        res["roofline_infer"] = {
+            "bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_BF16_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 4),
+            "kernel": "featurize_kernel + mlp_infer_kernel (404,480 FLOP per cell the network "
+                      "runs on)"}
+        res["model"] = os.path.relpath(path, ROOT)
+        mlp.close()
     return res
 
 

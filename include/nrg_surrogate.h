@@ -59,6 +59,23 @@ extern "C" {
 /* per-cell status of an accepted surrogate answer (extends NRG_CELL_*) */
 #define NRG_CELL_SURROGATE 6
 
+/*
+ * gate.screen: which cells the network sees.
+ *   NONE   every live cell (the gate's checks decide, as listed above);
+ *   RATES  an explicit-rate screen runs first: a live cell whose explicit
+ *          estimate of the step, dY_s = omega_s(C, T) dt W_s / rho, has every
+ *          |dY_s| <= dY_freeze and implies |dT| <= dT_freeze (at the cell's
+ *          frozen cv) is accepted with the zero increment without the network;
+ *          the network and the gate then run on the other live cells only;
+ *   ONLY   the same screen, and every cell it does not freeze goes to the
+ *          FP64 integrator: the method without the network (mlp and norm may
+ *          be NULL, p is not read).
+ * RATES and ONLY need dT_freeze > 0.
+ */
+#define NRG_SUR_SCREEN_NONE 0
+#define NRG_SUR_SCREEN_RATES 1
+#define NRG_SUR_SCREEN_ONLY 2
+
 typedef struct nrg_surrogate_norm {
   float in_shift[NRG_SUR_MAX_IN], in_scale[NRG_SUR_MAX_IN];     /* S + 3 used */
   float out_shift[NRG_SUR_MAX_OUT], out_scale[NRG_SUR_MAX_OUT]; /* S + 1 used */
@@ -74,7 +91,7 @@ typedef struct nrg_surrogate_gate {
   double dT_max;                                     /* K                              */
   double dT_freeze, dY_freeze;                       /* K, mass fraction (dT_freeze 0: off) */
   int32_t n_elements;                                /* rows of `elements` (<= 4)      */
-  int32_t reserved0;
+  int32_t screen;                                    /* NRG_SUR_SCREEN_*, below        */
   double rate_tol;                                   /* mass fraction (0: off), below  */
   int32_t reserved[4];
   double elements[NRG_SUR_MAX_ELEMENTS][32];         /* atoms of element e in species s */
@@ -89,6 +106,8 @@ typedef struct nrg_surrogate_stats {
   double fallback_ms;         /* FP64 integration of the rejected cells                */
   nrg_chem_stats fallback;    /* counters of that integration                          */
   int64_t reject_rate;        /* rejected by the rate-consistency test (rate_tol)      */
+  int64_t cells_network;      /* cells the network ran on                              */
+  double screen_ms;           /* the explicit-rate screen (SCREEN_RATES, SCREEN_ONLY)   */
 } nrg_surrogate_stats;
 
 /*

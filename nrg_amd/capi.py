@@ -48,6 +48,9 @@ NRG_CELL_SURROGATE = 6          # include/nrg_surrogate.h
 NRG_SUR_MAX_IN = 16
 NRG_SUR_MAX_OUT = 16
 NRG_SUR_MAX_ELEMENTS = 4
+NRG_SUR_SCREEN_NONE = 0         # gate.screen: the network sees every live cell
+NRG_SUR_SCREEN_RATES = 1        # explicit-rate screen first, network on the rest
+NRG_SUR_SCREEN_ONLY = 2         # explicit-rate screen, FP64 on the rest (no network)
 
 
 class NrgChemMech(C.Structure):
@@ -145,7 +148,7 @@ class NrgSurrogateGate(C.Structure):
                 ("elem_tol", C.c_double), ("elem_floor", C.c_double), ("y_neg_tol", C.c_double),
                 ("energy_tol", C.c_double), ("dT_max", C.c_double),
                 ("dT_freeze", C.c_double), ("dY_freeze", C.c_double),
-                ("n_elements", C.c_int32), ("reserved0", C.c_int32), ("rate_tol", C.c_double),
+                ("n_elements", C.c_int32), ("screen", C.c_int32), ("rate_tol", C.c_double),
                 ("reserved", C.c_int32 * 4),
                 ("elements", (C.c_double * 32) * NRG_SUR_MAX_ELEMENTS)]
 
@@ -158,7 +161,8 @@ class NrgSurrogateStats(C.Structure):
                 ("reject_negative", C.c_int64), ("reject_energy", C.c_int64),
                 ("reject_confidence", C.c_int64), ("cells_frozen", C.c_int64),
                 ("infer_ms", C.c_double), ("gate_ms", C.c_double), ("fallback_ms", C.c_double),
-                ("fallback", NrgChemStats), ("reject_rate", C.c_int64)]
+                ("fallback", NrgChemStats), ("reject_rate", C.c_int64),
+                ("cells_network", C.c_int64), ("screen_ms", C.c_double)]
 
 
 class NrgChemTuneTrial(C.Structure):

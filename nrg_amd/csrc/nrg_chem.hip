@@ -916,7 +916,7 @@ __global__ void __launch_bounds__(BIN_THREADS) bin_scatter_kernel(BinArgs a)
 constexpr int NIN = S + 3, NOUT = S + 1;
 
 enum { SUR_FALLBACK = 0, SUR_ACCEPT, SUR_GATED, SUR_R_DOMAIN, SUR_R_ELEM, SUR_R_NEG, SUR_R_ENERGY,
-       SUR_R_CONF, SUR_R_RATE, SUR_FROZEN, SUR_SLOTS = 10 };
+       SUR_R_CONF, SUR_R_RATE, SUR_FROZEN, SUR_CAND, SUR_SLOTS = 11 };
 constexpr int SUR_NREASONS = SUR_R_RATE - SUR_R_DOMAIN + 1;
 
 struct SurArgs {
@@ -933,8 +933,12 @@ struct SurArgs {
   double elem_tol, elem_floor, y_neg_tol, energy_tol, dT_max, dT_freeze, dY_freeze, rate_tol;
   int ne;
   double E[NRG_SUR_MAX_ELEMENTS][S];
-  bf16 *X;                    // ninner x MLP_IN_PAD
-  const float *Ypred;         // ninner x NOUT
+  const int32_t *cand;        // NULL: position p is inner cell p; else inner cell cand[p]
+  long long npos;             // positions the network, rate check and gate visit
+  int32_t *screen_out;        // rate_screen_kernel: where the cells it does not freeze go
+  int screen_slot;            // ... and the counter that numbers them (SUR_CAND, SUR_FALLBACK)
+  bf16 *X;                    // npos x MLP_IN_PAD
+  const float *Ypred;         // npos x NOUT
   double *Yp[S], *Yp2[S];
   double *Efp;
   int32_t *status;
@@ -951,10 +955,11 @@ NRG_DI bool gated_cell(const SurArgs &a, long long off)
 
 __global__ void __launch_bounds__(256) featurize_kernel(SurArgs a)
 {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= a.g.ninner) return;
+  const long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= a.npos) return;
+  const long long q = a.cand ? (long long)a.cand[pos] : pos;
   const long long off = cell_offset(a.g, q);
-  bf16 *x = a.X + q * MLP_IN_PAD;
+  bf16 *x = a.X + pos * MLP_IN_PAD;
   float f[MLP_IN_PAD];
 #pragma unroll
   for (int i = 0; i < MLP_IN_PAD; ++i) f[i] = 0.0f;
@@ -1053,9 +1058,10 @@ __global__ void __launch_bounds__(256) rate_check_kernel(SurArgs a)
   stage_mech(mc, a.mech);
   const int group = __builtin_amdgcn_readfirstlane(threadIdx.x / RCHECK_CELLS);
   const int local = threadIdx.x % RCHECK_CELLS;
-  const long long q = (long long)blockIdx.x * RCHECK_CELLS + local;
-  const long long off = q < a.g.ninner ? cell_offset(a.g, q) : 0;
-  const bool live = q < a.g.ninner && !gated_cell(a, off);
+  const long long pos = (long long)blockIdx.x * RCHECK_CELLS + local;
+  const long long q = pos < a.npos ? (a.cand ? (long long)a.cand[pos] : pos) : 0;
+  const long long off = pos < a.npos ? cell_offset(a.g, q) : 0;
+  const bool live = pos < a.npos && !gated_cell(a, off);
   double y[S], C[S], Zd[SA];
   double rho = 1.0;
   if (live) {
@@ -1073,10 +1079,10 @@ __global__ void __launch_bounds__(256) rate_check_kernel(SurArgs a)
     }
   }
   __syncthreads();
-  if (group != 0 || q >= a.g.ninner) return;
+  if (group != 0 || pos >= a.npos) return;
   int8_t bad = 0;
   if (live) {
-    const float *yp = a.Ypred + q * NOUT;
+    const float *yp = a.Ypred + pos * NOUT;
     const double dT = (double)yp[0] * a.out_scale[0] + a.out_shift[0];
     double dy[S], dymax = 0.0;
 #pragma unroll
@@ -1092,7 +1098,101 @@ __global__ void __launch_bounds__(256) rate_check_kernel(SurArgs a)
       if (!(fabs(est - (frozen ? 0.0 : dy[s])) <= a.rate_tol)) bad = 1;
     }
   }
-  a.rate_bad[q] = bad;
+  a.rate_bad[pos] = bad;
+}
+
+// The explicit-rate screen: the learned step's freeze test without the
+// network.  For every live cell it forms the explicit estimate of the step,
+// dY_s = omega_s(C, T) dt W_s / rho (rate constants and mass action as in
+// rate_check_kernel), and the temperature change that estimate implies at the
+// cell's frozen cv, dT = -(sum_s e_s(T) dY_s / W_s) / cv with
+// cv = sum_s (Y_s / W_s)(cp_s(T) - R).  A cell with every |dY_s| <= dY_freeze
+// and |dT| <= dT_freeze is taken as inert and gets the zero increment (the FP64
+// step's outputs for a cell whose concentrations do not change).  Every other
+// live cell is appended to screen_out: the network's candidates, or, for the
+// method without the network, the FP64 fallback list.  Gated cells get the
+// FP64 step's outputs for a skipped cell (chemical_kinetics_solver.f90:341-346).
+__global__ void __launch_bounds__(256) rate_screen_kernel(SurArgs a)
+{
+  __shared__ MechCoef mc;
+  __shared__ double part[RCHECK_CELLS][SA];
+  __shared__ unsigned long long blk[SUR_SLOTS];
+  if (threadIdx.x < SUR_SLOTS) blk[threadIdx.x] = 0ull;
+  stage_mech(mc, a.mech);
+  const int group = __builtin_amdgcn_readfirstlane(threadIdx.x / RCHECK_CELLS);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int local = threadIdx.x % RCHECK_CELLS;
+  const long long q = (long long)blockIdx.x * RCHECK_CELLS + local;
+  const bool valid = q < a.g.ninner;
+  const long long off = valid ? cell_offset(a.g, q) : 0;
+  const bool gated = valid && gated_cell(a, off);
+  const bool live = valid && !gated;
+  double C[S], Zd[SA];
+  double rho = 1.0, T = 0.0;
+  if (live) {
+    rho = a.rho[off];
+    T = a.T[off];
+#pragma unroll
+    for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) / mc.W[s];
+    if (group == 0) explicit_rates<EstimateMath, 0, R / 2>(mc, T, C, Zd);
+    else explicit_rates<EstimateMath, R / 2, R>(mc, T, C, Zd);
+    if (group == 1) {
+#pragma unroll
+      for (int i = 0; i < SA; ++i) part[local][i] = Zd[i];
+    }
+  }
+  __syncthreads();
+  if (group == 0) {
+    bool frozen = false;
+    if (live) {
+      double dymax = 0.0, de = 0.0, cv = 0.0;
+#pragma unroll
+      for (int i = 0; i < SA; ++i) {
+        const int s = kSp.full[i];
+        const double est = (((Zd[i] + part[local][i]) * a.dt) * mc.W[s]) / rho;
+        dymax = fmax(dymax, fabs(est));
+        de += e_molar(mc, T, s) * (est / mc.W[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s) cv += (C[s] / rho) * (cp_molar(mc, T, s) - R_GAS_J);
+      const double dT = -de / cv;
+      frozen = a.dT_freeze > 0.0 && dymax <= a.dY_freeze && fabs(dT) <= a.dT_freeze;
+      if (frozen) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          a.Yp[s][off] = 0.0;
+          a.Yp2[s][off] = 0.0;
+        }
+        a.Efp[off] = 0.0;
+        if (a.status) a.status[off] = NRG_CELL_SURROGATE;
+      }
+    }
+    if (gated) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
+      a.Efp[off] = 0.0;
+      if (a.status) a.status[off] = NRG_CELL_GATED;
+    }
+    const bool pass = live && !frozen;
+    const unsigned long long mask = __ballot(pass ? 1 : 0);
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(&a.acc[a.screen_slot], (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader, WAVE);
+      if (pass) a.screen_out[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)q;
+    }
+    const unsigned long long c_frz = __popcll(__ballot(live && frozen ? 1 : 0));
+    const unsigned long long c_gat = __popcll(__ballot(gated ? 1 : 0));
+    if (lane == 0) {
+      atomicAdd(&blk[SUR_ACCEPT], c_frz);
+      atomicAdd(&blk[SUR_FROZEN], c_frz);
+      atomicAdd(&blk[SUR_GATED], c_gat);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < SUR_SLOTS && (int)threadIdx.x != a.screen_slot && blk[threadIdx.x])
+    atomicAdd(&a.acc[threadIdx.x], blk[threadIdx.x]);
 }
 
 template <bool RATE>
@@ -1103,8 +1203,9 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
   if (threadIdx.x < SUR_SLOTS) blk[threadIdx.x] = 0ull;
   stage_mech(mc, a.mech);
   const int lane = threadIdx.x & (WAVE - 1);
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = q < a.g.ninner;
+  const long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = pos < a.npos;
+  const long long q = valid ? (a.cand ? (long long)a.cand[pos] : pos) : 0;
   const long long off = valid ? cell_offset(a.g, q) : 0;
   const bool gated = valid && gated_cell(a, off);
   const bool live = valid && !gated;
@@ -1114,7 +1215,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
     double y[S], dy[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) y[s] = a.Y[s][off];
-    const float *yp = a.Ypred + q * NOUT;
+    const float *yp = a.Ypred + pos * NOUT;
     const double dT = (double)yp[0] * a.out_scale[0] + a.out_shift[0];
     bool finite = isfinite(dT);
 #pragma unroll
@@ -1131,7 +1232,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
       for (int s = 0; s < S; ++s) dymax = fmax(dymax, fabs(dy[s]));
       frozen = a.dT_freeze > 0.0 && fabs(dT) <= a.dT_freeze && dymax <= a.dY_freeze;
       if (!frozen && !(fabs(dT) <= a.dT_max)) reason = SUR_R_CONF;
-      if (RATE && reason == 0 && a.rate_bad[q]) {
+      if (RATE && reason == 0 && a.rate_bad[pos]) {
         frozen = false;
         reason = SUR_R_RATE;
       }
@@ -1324,6 +1425,7 @@ struct nrg_chem_handle {
   nrg::bf16 *sur_X = nullptr;
   float *sur_Y = nullptr;
   int32_t *sur_list = nullptr;
+  int32_t *sur_cand = nullptr;             // cells the explicit-rate screen hands to the network
   int8_t *sur_rate_bad = nullptr;
   unsigned long long *sur_acc = nullptr;
   long long sur_cells = 0;
@@ -1677,7 +1779,7 @@ void nrg_chem_destroy(nrg_chem_t *h)
   if (h->ev2) hipEventDestroy(h->ev2);
   if (h->ev3) hipEventDestroy(h->ev3);
   for (void *p : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_rate_bad,
-                  (void *)h->sur_acc,
+                  (void *)h->sur_cand, (void *)h->sur_acc,
                   (void *)h->order_list, (void *)h->hist, (void *)h->bins, (void *)h->table})
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -2511,11 +2613,13 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
                                 double *const *Y_prod, double *const *Y_prod2, double *E_f_prod,
                                 int32_t *cell_status, nrg_surrogate_stats *stats)
 {
-  if (!h || !mlp || !norm || !gate) return set_error(NRG_CHEM_E_ARG, "handle, network, norm or gate is NULL");
+  if (!h || !gate) return set_error(NRG_CHEM_E_ARG, "handle or gate is NULL");
+  const bool use_net = gate->screen != NRG_SUR_SCREEN_ONLY;
+  if (use_net && (!mlp || !norm)) return set_error(NRG_CHEM_E_ARG, "network or norm is NULL");
   if (!(dt > 0.0) || !std::isfinite(dt)) return set_error(NRG_CHEM_E_ARG, "time_step must be > 0");
-  if (!T || !p || !rho || !Y || !Y_prod || !Y_prod2 || !E_f_prod)
+  if (!T || !rho || !Y || !Y_prod || !Y_prod2 || !E_f_prod || (use_net && !p))
     return set_error(NRG_CHEM_E_ARG, "required field pointer is NULL (the network needs p)");
-  if (mlp_n_in(mlp) != NIN || mlp_n_out(mlp) != NOUT)
+  if (use_net && (mlp_n_in(mlp) != NIN || mlp_n_out(mlp) != NOUT))
     return set_error(NRG_CHEM_E_ARG, "network maps %d -> %d, the step needs %d -> %d", mlp_n_in(mlp),
                      mlp_n_out(mlp), NIN, NOUT);
   if (gate->n_elements < 0 || gate->n_elements > NRG_SUR_MAX_ELEMENTS)
@@ -2544,15 +2648,17 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   a.acc = h->dev_acc;
   a.jac_pct = h->jac_pct;
   sa.t_gate = h->k.t_gate;
-  for (int i = 0; i < NIN; ++i) {
-    sa.in_shift[i] = norm->in_shift[i];
-    sa.in_scale[i] = norm->in_scale[i];
+  if (norm) {
+    for (int i = 0; i < NIN; ++i) {
+      sa.in_shift[i] = norm->in_shift[i];
+      sa.in_scale[i] = norm->in_scale[i];
+    }
+    for (int i = 0; i < NOUT; ++i) {
+      sa.out_shift[i] = norm->out_shift[i];
+      sa.out_scale[i] = norm->out_scale[i];
+    }
+    sa.y_floor = norm->y_floor;
   }
-  for (int i = 0; i < NOUT; ++i) {
-    sa.out_shift[i] = norm->out_shift[i];
-    sa.out_scale[i] = norm->out_scale[i];
-  }
-  sa.y_floor = norm->y_floor;
   sa.t_min = gate->t_min; sa.t_max = gate->t_max;
   sa.p_min = gate->p_min; sa.p_max = gate->p_max;
   sa.dt_ok = dt >= gate->dt_min && dt <= gate->dt_max;
@@ -2561,6 +2667,11 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   sa.dT_freeze = gate->dT_freeze; sa.dY_freeze = gate->dY_freeze;
   if (!(gate->rate_tol >= 0.0)) return set_error(NRG_CHEM_E_ARG, "gate rate_tol must be >= 0");
   sa.rate_tol = gate->rate_tol;
+  const int screen = gate->screen;
+  if (screen != NRG_SUR_SCREEN_NONE && screen != NRG_SUR_SCREEN_RATES && screen != NRG_SUR_SCREEN_ONLY)
+    return set_error(NRG_CHEM_E_ARG, "unknown gate screen %d", screen);
+  if (screen != NRG_SUR_SCREEN_NONE && !(gate->dT_freeze > 0.0))
+    return set_error(NRG_CHEM_E_ARG, "the explicit-rate screen needs dT_freeze > 0");
   sa.ne = gate->n_elements;
   for (int e = 0; e < sa.ne; ++e)
     for (int s = 0; s < S; ++s) sa.E[e][s] = gate->elements[e][s];
@@ -2568,14 +2679,17 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   HIP_TRY(hipSetDevice(h->device));
   const long long n = a.g.ninner;
   if (n > h->sur_cells) {
-    for (void *q : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_rate_bad})
+    for (void *q : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_rate_bad,
+                    (void *)h->sur_cand})
       if (q) HIP_TRY(hipFree(q));
     h->sur_X = nullptr; h->sur_Y = nullptr; h->sur_list = nullptr; h->sur_rate_bad = nullptr;
+    h->sur_cand = nullptr;
     h->sur_cells = 0;
     HIP_TRY(hipMalloc(&h->sur_X, (size_t)n * MLP_IN_PAD * sizeof(bf16)));
     HIP_TRY(hipMalloc(&h->sur_Y, (size_t)n * NOUT * sizeof(float)));
     HIP_TRY(hipMalloc(&h->sur_list, (size_t)n * sizeof(int32_t)));
     HIP_TRY(hipMalloc(&h->sur_rate_bad, (size_t)n * sizeof(int8_t)));
+    HIP_TRY(hipMalloc(&h->sur_cand, (size_t)n * sizeof(int32_t)));
     h->sur_cells = n;
   }
   if (!h->sur_acc) HIP_TRY(hipMalloc(&h->sur_acc, SUR_SLOTS * sizeof(unsigned long long)));
@@ -2584,29 +2698,59 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   sa.list = h->sur_list;
   sa.rate_bad = h->sur_rate_bad;
   sa.acc = h->sur_acc;
+  sa.cand = nullptr;
+  sa.npos = n;
 
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(ACC_SLOTS), 0, h->stream, h->dev_acc, ACC_SLOTS);
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(SUR_SLOTS), 0, h->stream, h->sur_acc, SUR_SLOTS);
   HIP_TRY(hipGetLastError());
   unsigned long long sur[SUR_SLOTS] = {}, acc[16] = {};
-  float ms_inf = 0.f, ms_gate = 0.f, ms_fb = 0.f;
+  float ms_scr = 0.f, ms_inf = 0.f, ms_gate = 0.f, ms_fb = 0.f;
+  long long ncand = n;
   if (n > 0) {
-    const unsigned nb = (unsigned)((n + 255) / 256);
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    hipLaunchKernelGGL(featurize_kernel, dim3(nb), dim3(256), 0, h->stream, sa);
-    HIP_TRY(hipGetLastError());
-    if ((st = mlp_infer_bf16(mlp, n, h->sur_X, h->sur_Y, h->stream))) return st;
-    HIP_TRY(hipEventRecord(h->ev2, h->stream));
-    if (sa.rate_tol > 0.0) {
-      hipLaunchKernelGGL(rate_check_kernel, dim3((unsigned)((n + RCHECK_CELLS - 1) / RCHECK_CELLS)),
+    if (screen != NRG_SUR_SCREEN_NONE) {
+      // the explicit-rate screen freezes the inert cells; the others go to the
+      // network (SCREEN_RATES) or straight to the FP64 integrator (SCREEN_ONLY)
+      sa.screen_out = screen == NRG_SUR_SCREEN_RATES ? h->sur_cand : h->sur_list;
+      sa.screen_slot = screen == NRG_SUR_SCREEN_RATES ? SUR_CAND : SUR_FALLBACK;
+      hipLaunchKernelGGL(rate_screen_kernel, dim3((unsigned)((n + RCHECK_CELLS - 1) / RCHECK_CELLS)),
                          dim3(256), 0, h->stream, sa);
-      hipLaunchKernelGGL(gate_kernel<true>, dim3(nb), dim3(256), 0, h->stream, sa);
-    } else
-      hipLaunchKernelGGL(gate_kernel<false>, dim3(nb), dim3(256), 0, h->stream, sa);
-    HIP_TRY(hipGetLastError());
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(sur, h->sur_acc, sizeof(sur), hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      ncand = screen == NRG_SUR_SCREEN_RATES ? (long long)sur[SUR_CAND] : 0;
+      if (ncand > n || (long long)sur[SUR_FALLBACK] > n)
+        return set_error(NRG_CHEM_E_HIP, "screen counts exceed %lld cells", n);
+      sa.cand = h->sur_cand;
+      sa.npos = ncand;
+    }
+    HIP_TRY(hipEventRecord(h->ev2, h->stream));
+    if (ncand > 0) {
+      const unsigned nb = (unsigned)((ncand + 255) / 256);
+      hipLaunchKernelGGL(featurize_kernel, dim3(nb), dim3(256), 0, h->stream, sa);
+      HIP_TRY(hipGetLastError());
+      if ((st = mlp_infer_bf16(mlp, ncand, h->sur_X, h->sur_Y, h->stream))) return st;
+    }
     HIP_TRY(hipEventRecord(h->ev3, h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev3));
+    HIP_TRY(hipEventElapsedTime(&ms_scr, h->ev0, h->ev2));
+    HIP_TRY(hipEventElapsedTime(&ms_inf, h->ev2, h->ev3));
+    if (ncand > 0) {
+      const unsigned nb = (unsigned)((ncand + 255) / 256);
+      if (sa.rate_tol > 0.0) {
+        hipLaunchKernelGGL(rate_check_kernel, dim3((unsigned)((ncand + RCHECK_CELLS - 1) / RCHECK_CELLS)),
+                           dim3(256), 0, h->stream, sa);
+        hipLaunchKernelGGL(gate_kernel<true>, dim3(nb), dim3(256), 0, h->stream, sa);
+      } else
+        hipLaunchKernelGGL(gate_kernel<false>, dim3(nb), dim3(256), 0, h->stream, sa);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(h->ev2, h->stream));
     HIP_TRY(hipMemcpyAsync(sur, h->sur_acc, sizeof(sur), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipEventElapsedTime(&ms_gate, h->ev3, h->ev2));
+    HIP_TRY(hipEventRecord(h->ev3, h->stream));
     const long long nfb = (long long)sur[SUR_FALLBACK];
     if (nfb > n) return set_error(NRG_CHEM_E_HIP, "fallback count %lld exceeds %lld cells", nfb, n);
     if (nfb > 0) {
@@ -2625,8 +2769,6 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipMemcpyAsync(acc, h->dev_acc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipEventElapsedTime(&ms_inf, h->ev0, h->ev2));
-    HIP_TRY(hipEventElapsedTime(&ms_gate, h->ev2, h->ev3));
     HIP_TRY(hipEventElapsedTime(&ms_fb, h->ev3, h->ev1));
   }
   if (stats) {
@@ -2642,6 +2784,8 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     stats->reject_confidence = (int64_t)sur[SUR_R_CONF];
     stats->reject_rate = (int64_t)sur[SUR_R_RATE];
     stats->cells_frozen = (int64_t)sur[SUR_FROZEN];
+    stats->cells_network = ncand;
+    stats->screen_ms = ms_scr;
     stats->infer_ms = ms_inf;
     stats->gate_ms = ms_gate;
     stats->fallback_ms = ms_fb;

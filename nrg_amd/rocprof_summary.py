@@ -2,13 +2,17 @@
 
     python -m nrg_amd.rocprof_summary stats   <kernel_stats.csv>
     python -m nrg_amd.rocprof_summary trace   <kernel_trace.csv> --last N
-    python -m nrg_amd.rocprof_summary counter <counter_collection.csv> [--calib-mib M]
+    python -m nrg_amd.rocprof_summary counter <counter_collection.csv> [--calib-mib M] [--last N]
 
 ``stats`` prints calls / average / total duration per kernel.  ``trace``
 prints, per kernel, the mean duration of its last N dispatches (the timed
 steps, after warm-up) from the per-dispatch trace.  ``counter``
 prints, per kernel, the average per-dispatch value of each collected counter
-(FETCH_SIZE and WRITE_SIZE are in KiB per dispatch).  With ``--calib-mib`` the
+(FETCH_SIZE and WRITE_SIZE are in KiB per dispatch), over its last N dispatches
+with ``--last N`` (the timed steps), else over all of them.  With
+SQ_THREAD_CYCLES_VALU and SQ_ACTIVE_INST_VALU both collected, a line
+    LANES <tab> kernel <tab> share of the 64 lanes busy per VALU cycle
+follows for each kernel.  With ``--calib-mib`` the
 largest single device-copy dispatch (the known-size copy bench.py runs with
 --copy-calibration-mib) is taken as the reference: its expected bytes divided
 by its measured bytes gives the factor applied to the other kernels.  Each
@@ -52,20 +56,27 @@ def trace(path, last):
         print("%-72s %6d %14.3f" % (_short(k), len(ds), sum(tail) / len(tail) / 1e6))
 
 
-def counters(path, calib_mib=None):
+def counters(path, calib_mib=None, last=None):
     rows = list(csv.DictReader(open(path)))
     name_key = "Kernel_Name" if "Kernel_Name" in rows[0] else "Kernel-Name"
-    acc = defaultdict(lambda: defaultdict(float))
-    disp = defaultdict(set)
     per_disp = defaultdict(float)          # (kernel, counter, dispatch) -> value
     for r in rows:
         k = r[name_key]
         cname = r.get("Counter_Name") or r.get("Counter-Name")
         did = r.get("Dispatch_Id") or r.get("Dispatch-Id")
         val = float(r.get("Counter_Value") or r.get("Counter-Value"))
-        acc[k][cname] += val
-        disp[k].add(did)
         per_disp[(k, cname, did)] += val
+    # each kernel's dispatches in launch order; with --last only the last N count
+    kd = defaultdict(set)
+    for (k, _, did) in per_disp:
+        kd[k].add(int(did))
+    keep = {k: set(sorted(v)[-last:] if last else v) for k, v in kd.items()}
+    acc = defaultdict(lambda: defaultdict(float))
+    disp = defaultdict(set)
+    for (k, cname, did), val in per_disp.items():
+        if int(did) in keep[k]:
+            acc[k][cname] += val
+            disp[k].add(did)
     factor = {}
     if calib_mib:
         # bench.py's known-size device-to-device copy runs as the runtime's
@@ -96,6 +107,12 @@ def counters(path, calib_mib=None):
                                                            v / nd * f * 1024.0))
     for t in traffic:
         print(t)
+    for k in sorted(acc, key=lambda k: -sum(acc[k].values())):
+        a = acc[k]
+        if a.get("SQ_ACTIVE_INST_VALU") and "SQ_THREAD_CYCLES_VALU" in a:
+            m = re.search(r"([A-Za-z_]\w*_kernel)\b", k)
+            print("LANES\t%s\t%.4f" % (m.group(1) if m else _short(k, 40),
+                                        a["SQ_THREAD_CYCLES_VALU"] / (64.0 * a["SQ_ACTIVE_INST_VALU"])))
 
 
 def main(argv):
@@ -111,7 +128,8 @@ def main(argv):
         calib = None
         if "--calib-mib" in argv:
             calib = float(argv[argv.index("--calib-mib") + 1])
-        counters(argv[1], calib)
+        last = int(argv[argv.index("--last") + 1]) if "--last" in argv else None
+        counters(argv[1], calib, last)
     else:
         print(__doc__)
         return 2

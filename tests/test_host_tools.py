@@ -81,3 +81,28 @@ def test_set_gate_rewrites_only_the_thresholds(tmp_path):
     import pytest
     with pytest.raises(KeyError):
         train_surrogate.main(["--out", str(out), "--set-gate", str(src), "--gate", "nope=1"])
+
+
+def test_counter_summary_last_dispatches_and_lane_share(tmp_path):
+    """--last N averages each kernel's last N dispatches only (the timed steps),
+    and the lane share is SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)."""
+    from nrg_amd import rocprof_summary as rs
+    p = tmp_path / "sq.csv"
+    k = "void (anonymous namespace)::chem_bdf_kernel<true, false>((anonymous namespace)::StepArgs)"
+    rows = [(k, "SQ_ACTIVE_INST_VALU", 1, 1000.0), (k, "SQ_THREAD_CYCLES_VALU", 1, 64000.0),
+            (k, "SQ_ACTIVE_INST_VALU", 7, 100.0), (k, "SQ_THREAD_CYCLES_VALU", 7, 1600.0),
+            (k, "SQ_ACTIVE_INST_VALU", 9, 300.0), (k, "SQ_THREAD_CYCLES_VALU", 9, 4800.0)]
+    with open(p, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kernel_Name", "Counter_Name", "Dispatch_Id", "Counter_Value"])
+        for r in rows:
+            w.writerow(r)
+    buf = io.StringIO()
+    with redirect_stdout(buf):
+        rs.counters(str(p), last=2)
+    out = buf.getvalue()
+    lanes = [l.split("\t") for l in out.splitlines() if l.startswith("LANES")]
+    # dispatches 7 and 9 only: (1600 + 4800) / (64 * 400) = 0.25
+    assert lanes == [["LANES", "chem_bdf_kernel", "0.2500"]]
+    avg = [l.split() for l in out.splitlines() if "SQ_ACTIVE_INST_VALU" in l]
+    assert avg and float(avg[0][-1]) == 200.0 and avg[0][-3] == "2"

@@ -61,7 +61,9 @@ def _run(engine, net, norm, gate, st, bc, dt):
     stats = capi.NrgSurrogateStats()
     torch.cuda.synchronize()
     rc = engine.lib.nrg_chem_surrogate_step_dev(
-        engine.h, net.h, C.byref(norm.struct()), C.byref(gate), C.byref(capi.flat_grid(n)), dt,
+        engine.h, net.h if net is not None else None,
+        C.byref(norm.struct()) if norm is not None else None, C.byref(gate),
+        C.byref(capi.flat_grid(n)), dt,
         st["T"].data_ptr(), st["p"].data_ptr(), st["rho"].data_ptr(),
         capi.ptr_array([Y[s].data_ptr() for s in range(S)]), bc.data_ptr(),
         capi.ptr_array([Yp[s].data_ptr() for s in range(S)]),
@@ -234,3 +236,77 @@ def test_rate_consistency_rejects_what_the_explicit_step_contradicts(engine, ora
     for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
         assert np.array_equal(o[k][..., rej], f[k][..., rej]), k
     assert np.all(o["Y_prod"][:, live & ~rej] == 0)
+
+
+def _explicit_estimate(oracle, mech, st, dt):
+    """Per cell: max_s |omega_s(C, T) dt W_s / rho| and the temperature change the
+    estimate implies at frozen cv, -sum_s e_s(T) dY_s / W_s / cv, from the oracle."""
+    T, rho, Y = st["T"].cpu().numpy(), st["rho"].cpu().numpy(), st["Y"].cpu().numpy()
+    W = np.asarray(mech.molar_masses)
+    R = 8.31446261815324
+    est, dT = np.zeros(T.size), np.zeros(T.size)
+    for q in range(T.size):
+        if T[q] < 305.0:
+            continue
+        Mc = oracle.mixture_concentration(Y[:, q], rho[q])
+        kf, kr = oracle.rate_constants(T[q], Mc)
+        dY = oracle.rhs(kf, kr, Mc, Y[:, q] * rho[q] / W) * dt * W / rho[q]
+        est[q] = np.abs(dY).max()
+        cv = sum(Y[s, q] / W[s] * (oracle.cp(T[q], s) - R) for s in range(S))
+        dT[q] = -sum(oracle.e(T[q], s) * dY[s] / W[s] for s in range(S)) / cv
+    return est, dT
+
+
+def test_rate_screen_without_network_is_the_freeze_test_plus_fp64(engine, oracle, mech):
+    """gate.screen = ONLY, the method without the network: a live cell whose
+    explicit estimate stays within dY_freeze in every species and dT_freeze in
+    temperature gets the zero increment; every other live cell is the FP64
+    step's, bit for bit; the network is never run."""
+    st, bc = _state(mech, seed=21)
+    dt, ytol, ttol = 1e-7, 1e-6, 0.05
+    gate = _gate(mech, dY_freeze=ytol, dT_freeze=ttol, screen=capi.NRG_SUR_SCREEN_ONLY)
+    o = _run(engine, None, None, gate, st, bc, dt)
+    f = _fp64(engine, st, bc, dt)
+    est, dT = _explicit_estimate(oracle, mech, st, dt)
+    live = (st["T"].cpu().numpy() >= 305.0) & (bc.cpu().numpy() == 0)
+    frozen = live & (o["status"] == capi.NRG_CELL_SURROGATE)
+    fb = live & ~frozen
+    s = o["stats"]
+    _report("screen_only", live=int(live.sum()), frozen=int(frozen.sum()), fallback=s.cells_fallback,
+            network=s.cells_network)
+    assert s.cells_network == 0 and s.cells_fallback == fb.sum() and s.cells_frozen == frozen.sum()
+    assert 0 < frozen.sum() < live.sum()
+    # the oracle's estimate decides, away from the thresholds
+    clear_in = live & (est < 0.5 * ytol) & (np.abs(dT) < 0.5 * ttol)
+    clear_out = live & ((est > 2.0 * ytol) | (np.abs(dT) > 2.0 * ttol))
+    assert np.all(frozen[clear_in]) and not np.any(frozen[clear_out])
+    for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+        assert np.array_equal(o[k][..., ~frozen], f[k][..., ~frozen]), k
+    assert np.all(o["Y_prod"][:, frozen] == 0) and np.all(o["Y_prod2"][:, frozen] == 0)
+    assert np.all(o["E_f_prod"][frozen] == 0)
+
+
+def test_rate_screen_then_network_equals_network_on_every_cell(engine, mech, zero_net):
+    """gate.screen = RATES runs the network only on the cells the screen does not
+    freeze.  With a network that answers zero, the gate freezes a cell exactly
+    when its explicit estimate is within rate_tol, and the screen (dY_freeze <=
+    rate_tol) only takes cells the gate would freeze anyway: outputs, statuses
+    and the FP64 fallback equal those of the network run on every cell."""
+    st, bc = _state(mech, seed=22)
+    dt = 1e-7
+    kw = dict(dY_freeze=1e-6, dT_freeze=0.05, rate_tol=1e-5)
+    a = _run(engine, zero_net, _norm(np.zeros(S + 1)),
+             _gate(mech, screen=capi.NRG_SUR_SCREEN_RATES, **kw), st, bc, dt)
+    b = _run(engine, zero_net, _norm(np.zeros(S + 1)),
+             _gate(mech, screen=capi.NRG_SUR_SCREEN_NONE, **kw), st, bc, dt)
+    live = int(a["stats"].cells_inner - a["stats"].cells_gated)
+    _report("screen_rates", live=live, network=a["stats"].cells_network,
+            accepted=a["stats"].cells_accepted, fallback=a["stats"].cells_fallback)
+    # without the screen the network runs on every inner cell (gated ones included)
+    assert b["stats"].cells_network == b["stats"].cells_inner
+    assert 0 < a["stats"].cells_network < live
+    for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("cells_accepted", "cells_fallback", "cells_gated", "cells_frozen"):
+        assert getattr(a["stats"], k) == getattr(b["stats"], k), k
+    assert a["stats"].fallback.nfe_sum == b["stats"].fallback.nfe_sum
