@@ -351,6 +351,16 @@ int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt,
 /* HIP stream the handle launches on (a hipStream_t), for callers that time or order work. */
 void *nrg_chem_stream(nrg_chem_t *h);
 
+/*
+ * Name of the reaction topology whose kernels serve the handle ("KEROMNES",
+ * "ZHANG"), or NULL for a NULL handle.  No counterpart in the reference, which
+ * interprets any mechanism at run time: here the kernels unroll a mechanism's
+ * reaction lists at compile time, one build per topology, and nrg_chem_create
+ * picks the build whose reaction types and species lists
+ * (chemical_properties_class.f90:15-52) equal the mechanism's.
+ */
+const char *nrg_chem_topology(nrg_chem_t *h);
+
 #ifdef __cplusplus
 }
 #endif

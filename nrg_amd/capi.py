@@ -332,6 +332,8 @@ def load_library(path=None):
     lib.nrg_energy_from_temperature_dev.restype = C.c_int
     lib.nrg_chem_stream.argtypes = [vp]
     lib.nrg_chem_stream.restype = vp
+    lib.nrg_chem_topology.argtypes = [vp]
+    lib.nrg_chem_topology.restype = C.c_char_p
     lib.nrg_chem_autotune_dev.argtypes = step_args[:-2] + [C.POINTER(NrgChemTuning)]
     lib.nrg_chem_autotune_dev.restype = C.c_int
     lib.nrg_chem_set_table.argtypes = [vp, C.c_int32, vp, vp]

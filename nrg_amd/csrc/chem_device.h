@@ -26,7 +26,11 @@
 
 #include <utility>
 
-#include "topology_keromnes.h"
+// the reaction topology these kernels are built for (one build per topology)
+#ifndef NRG_TOPOLOGY_HEADER
+#define NRG_TOPOLOGY_HEADER "topology_keromnes.h"
+#endif
+#include NRG_TOPOLOGY_HEADER
 
 #define NRG_DI __device__ __forceinline__
 

@@ -1,9 +1,23 @@
-// nrg_chem.hip -- kernels and C-ABI of the gfx950 chemistry step.
+// nrg_chem.hip -- kernels and entry points of the gfx950 chemistry step for
+// one reaction topology.
 //
 // Entry points are declared in include/nrg_chem.h; each replaces a piece of
 // NRG's chemical_kinetics_solver (computing_module/src/current_build/
 // chemical_kinetics_solver.f90) or of the split-step closure the flow solvers
 // apply after it.  All device work runs on the handle's stream.
+//
+// The kernels unroll the mechanism's reaction lists at compile time, so this
+// file is built once per topology (the Makefile sets NRG_TOPOLOGY_HEADER,
+// NRG_TOPO_NS, NRG_TOPO_OPS_FN, NRG_TOPO_NAME and NRG_TOPO_PRIMARY).  Each build
+// lives in its own namespace and hands its entry points to the C-ABI
+// (nrg_api.cpp) as a table (topo_api.h); nrg_chem_create picks the table whose
+// topology the mechanism has.
+#ifndef NRG_TOPO_NS
+#define NRG_TOPO_NS nrg_topo_keromnes
+#define NRG_TOPO_OPS_FN nrg_topo_ops_keromnes
+#define NRG_TOPO_NAME "KEROMNES"
+#define NRG_TOPO_PRIMARY 1
+#endif
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,23 +36,22 @@
 #include "chem_device.h"
 #include "mlp_internal.h"
 #include "nrg_common.h"
-
-namespace nrg {
-
-thread_local char g_last_error[512] = "";
-
-int set_error(int status, const char *fmt, ...)
-{
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
-  va_end(ap);
-  return status;
-}
-
-}  // namespace nrg
+#include "topo_api.h"
 
 using namespace nrg;
+
+// nothing of this build is exported: the C-ABI reaches it through its table
+#pragma GCC visibility push(hidden)
+namespace NRG_TOPO_NS {
+
+// This build's handle; the C-ABI sees it only as an opaque pointer.
+struct nrg_chem_handle;
+typedef nrg_chem_handle nrg_chem_t;
+void nrg_chem_destroy(nrg_chem_t *h);
+int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const double *T,
+                            const double *p, const double *rho, const double *const *Y,
+                            const int8_t *bc, double *const *Y_prod, double *const *Y_prod2,
+                            double *E_f_prod, int32_t *cell_status, nrg_chem_stats *stats);
 
 namespace {
 
@@ -1386,7 +1399,10 @@ static_assert(NOALLOC_ACC_BYTES + (size_t)NRG_NOALLOC_CHUNK_CELLS * (2 * R + NM)
                   (size_t)NRG_NOALLOC_SCRATCH_BYTES, "rate constants of one chunk");
 static_assert((long long)NRG_NOALLOC_MAX_CELLS <= (long long)MAX_CHUNKS * NRG_NOALLOC_CHUNK_CELLS,
               "one queue head per chunk");
+#if NRG_TOPO_PRIMARY
+// (one build of the library holds the array; the others refuse the no-alloc entry point)
 static __device__ __align__(256) unsigned char scratch[NRG_NOALLOC_SCRATCH_BYTES];
+#endif
 
 // ===========================================================================
 // Host side
@@ -1622,6 +1638,10 @@ static std::mutex g_noalloc_mu;
 
 static int noalloc_setup(int dev)
 {
+#if !NRG_TOPO_PRIMARY
+  (void)dev;
+  return NRG_CHEM_OK;
+#else
   if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
     return set_error(NRG_CHEM_E_NO_DEVICE, "device %d beyond the %d the library serves", dev,
                      NOALLOC_MAX_DEVICES);
@@ -1634,6 +1654,7 @@ static int noalloc_setup(int dev)
                      hipGetErrorString(e));
   g_noalloc_scratch[dev] = ptr;
   return NRG_CHEM_OK;
+#endif
 }
 
 // Blocks of 256 threads covering the aligned walk, one offset per thread.
@@ -1652,43 +1673,7 @@ static long long resident_blocks(const nrg_chem_handle *h, K kernel, size_t dyn_
   return std::min<long long>(walk_blocks(g), (long long)per_cu * h->num_cu);
 }
 
-extern "C" {
-
-int nrg_chem_abi_version(void) { return NRG_CHEM_ABI_VERSION; }
-
-void nrg_chem_default_opts(nrg_chem_opts *o)
-{
-  if (!o) return;
-  std::memset(o, 0, sizeof(*o));
-  o->eps = 1.0e-9;                    // chemical_kinetics_solver.f90:274
-  o->ewt = 1.0e-4;                    // :275
-  o->hmax = (double)1.0e-7f;          // :287, default-real literal
-  o->t_gate = 305.0;                  // :228
-  o->mxstep = 10000;                  // :288
-  o->mxord = 5;                       // :282
-  o->jacobian = NRG_JAC_FD_SLATEC;
-  o->device = 0;
-  o->form = NRG_FORM_PLAIN;
-  o->jac_batch_pct = 0;
-  o->waves_per_cu = 0;
-}
-
-const char *nrg_chem_strerror(int st)
-{
-  switch (st) {
-    case NRG_CHEM_OK: return "ok";
-    case NRG_CHEM_E_ARG: return "invalid argument";
-    case NRG_CHEM_E_MECH: return "mechanism not supported by the compiled kernels";
-    case NRG_CHEM_E_HIP: return "HIP runtime error";
-    case NRG_CHEM_E_NO_DEVICE: return "no gfx950 device";
-    case NRG_CHEM_E_FORMAT: return "unreadable mechanism, thermo or molar-mass file";
-    case NRG_CHEM_E_STEP_ZERO: return "integrator step size went to zero in a cell (SLATEC IERFLG 41)";
-    case NRG_CHEM_E_REPEATED_FAIL: return "integrator failed 50 attempts in a cell (SLATEC IERFLG 42)";
-    default: return "unknown status";
-  }
-}
-
-const char *nrg_chem_last_error(void) { return g_last_error; }
+// ---- entry points of this topology (forwarded from nrg_api.cpp) ----------
 
 int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_chem_t **out)
 {
@@ -2090,6 +2075,12 @@ int nrg_chem_step_dev_noalloc(nrg_chem_t *h, const nrg_grid *grid, double dt, co
                               double *E_f_prod, int32_t *cell_status, nrg_chem_stats *stats)
 {
   (void)p;
+#if !NRG_TOPO_PRIMARY
+  (void)h; (void)grid; (void)dt; (void)T; (void)rho; (void)Y; (void)bc; (void)Y_prod;
+  (void)Y_prod2; (void)E_f_prod; (void)cell_status; (void)stats;
+  return set_error(NRG_CHEM_E_MECH, "nrg_chem_step_dev_noalloc serves the KEROMNES kernels only, "
+                                    "not " NRG_TOPO_NAME);
+#else
   StepArgs a{};
   int st = fill_step_args(h, grid, dt, T, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status, a);
   if (st) return st;
@@ -2116,6 +2107,7 @@ int nrg_chem_step_dev_noalloc(nrg_chem_t *h, const nrg_grid *grid, double dt, co
   HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   if (stats) fill_stats(acc, a.g.ninner, ms, stats);
   return integrator_status(acc);
+#endif
 }
 
 // Host fields: stage them through the handle's device buffers, run the FP64
@@ -2794,4 +2786,45 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   return integrator_status(acc);
 }
 
-}  // extern "C"
+// ---- this topology's table for the C-ABI -------------------------------------
+#define H(h) reinterpret_cast<nrg_chem_t *>(h)
+#define NRG_TOPO_ADAPTER(name, params, args) static int op_##name params { return name args; }
+NRG_TOPO_ENTRY_POINTS(NRG_TOPO_ADAPTER)
+#undef NRG_TOPO_ADAPTER
+
+static int op_check(const nrg_chem_mech *mech)
+{
+  MechCoef tmp;
+  return build_mech(mech, tmp);
+}
+static int op_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, ::nrg_chem_t **out)
+{
+  nrg_chem_t *h = nullptr;
+  const int st = nrg_chem_create(mech, opts, &h);
+  *out = reinterpret_cast<::nrg_chem_t *>(h);
+  return st;
+}
+static void op_destroy(::nrg_chem_t *h) { nrg_chem_destroy(H(h)); }
+static void *op_stream(::nrg_chem_t *h) { return nrg_chem_stream(H(h)); }
+#undef H
+
+}  // namespace NRG_TOPO_NS
+#pragma GCC visibility pop
+
+__attribute__((visibility("hidden"))) const nrg_topo_ops *NRG_TOPO_OPS_FN()
+{
+  using namespace NRG_TOPO_NS;
+#define NRG_TOPO_OP_INIT(name, params, args) t.name = op_##name;
+  static const nrg_topo_ops table = [] {
+    nrg_topo_ops t{};
+    t.name = NRG_TOPO_NAME;
+    t.check = op_check;
+    t.create = op_create;
+    t.destroy = op_destroy;
+    t.stream = op_stream;
+    NRG_TOPO_ENTRY_POINTS(NRG_TOPO_OP_INIT)
+    return t;
+  }();
+#undef NRG_TOPO_OP_INIT
+  return &table;
+}

@@ -122,10 +122,12 @@ def run(shape=(512, 512, 256), steps=200, dt=1e-7, retrain_every=50, audit_frac=
         # write the chemistry outputs only, not the state
         tuning = {"learned": lrn.solver.autotune(dt).best.as_dict(),
                   "fp64": shd.solver.autotune(dt).best.as_dict()}
-    mlp, norm = sg.load(model)
+    # screen "only": the method without the network (explicit-rate freeze, FP64 for the rest)
+    mlp, norm = sg.load(model) if screen != "only" else (None, None)
     gkw = _gate_kwargs(model)
     gkw.update(gate_overrides or {})
-    scr = {"none": capi.NRG_SUR_SCREEN_NONE, "rates": capi.NRG_SUR_SCREEN_RATES}[screen]
+    scr = {"none": capi.NRG_SUR_SCREEN_NONE, "rates": capi.NRG_SUR_SCREEN_RATES,
+           "only": capi.NRG_SUR_SCREEN_ONLY}[screen]
     chem = sg.SurrogateChemistry(lrn.solver, mlp, norm, sg.gate_struct(mech, screen=scr, **gkw))
     n_aud = max(1, int(round(ncells * audit_frac)))
     gen = torch.Generator(device=dev)
@@ -234,7 +236,8 @@ def run(shape=(512, 512, 256), steps=200, dt=1e-7, retrain_every=50, audit_frac=
             "%.2e, max |dY| %.2e, p99 %.1e / %.1e" % (
                 k, rec["ms"], rec["fp64_ms"], rec["fallback_frac"], rec["network_frac"],
                 relT_max, dY_max, rec["sample_p99_rel_T"], rec["sample_p99_abs_Y_major"]))
-        if retrain_every > 0 and (k + 1) % retrain_every == 0 and (k + 1) < steps and pairs["f"]:
+        if (retrain_every > 0 and (k + 1) % retrain_every == 0 and (k + 1) < steps and pairs["f"]
+                and mlp is not None):
             fcat, dcat = torch.cat(pairs["f"]), torch.cat(pairs["d"])
             src = torch.cat(pairs["src"])
             x, y = norm.x(fcat), norm.y(dcat)
@@ -321,7 +324,8 @@ def run(shape=(512, 512, 256), steps=200, dt=1e-7, retrain_every=50, audit_frac=
         "field_GB": round(field_bytes / 1e9, 2),
         "retrains": retrains, "per_step": steps_log,
     }
-    mlp.close()
+    if mlp is not None:
+        mlp.close()
     for tr in (lrn, shd):
         tr.solver.engine.close()
     return rep
@@ -338,7 +342,8 @@ def main(argv=None):
     ap.add_argument("--retrain-lr", type=float, default=1e-4)
     ap.add_argument("--shift-max", type=int, nargs=2, default=[3, 2],
                     help="largest |shift| per step along x and y, in cells")
-    ap.add_argument("--screen", choices=["rates", "none"], default="rates")
+    ap.add_argument("--screen", choices=["rates", "none", "only"], default="rates",
+                    help="only: the method without the network")
     ap.add_argument("--model", default=None)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--gate", action="append", default=[], metavar="K=V",

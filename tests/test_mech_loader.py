@@ -160,13 +160,15 @@ def test_format_errors(tmp_path):
 
 
 def test_native_view_feeds_create():
-    """nrg_chem_create takes the loader's view: KEROMNES passes the topology
-    check (then needs a device), KONNOV is refused as a mechanism the compiled
-    kernels do not cover."""
+    """nrg_chem_create takes the loader's view: KEROMNES and ZHANG pass the
+    topology check of their kernel builds (then need a device); KONNOV, TEREZA
+    and ACETYLENE are refused as mechanisms no compiled build covers, with
+    each build's reason."""
     import torch
     lib = capi.load_library()
     opts = capi.default_opts()
-    for name, want in (("KEROMNES", None), ("KONNOV", capi.NRG_CHEM_E_MECH)):
+    for name, want in (("KEROMNES", None), ("ZHANG", None), ("KONNOV", capi.NRG_CHEM_E_MECH),
+                       ("TEREZA", capi.NRG_CHEM_E_MECH), ("ACETYLENE", capi.NRG_CHEM_E_MECH)):
         mech, thermo = CASES[name]
         h = C.c_void_p()
         capi.check(lib, lib.nrg_mech_load(mech.encode(), thermo.encode(), MM.encode(), 1.0,
@@ -182,6 +184,8 @@ def test_native_view_feeds_create():
                 lib.nrg_chem_destroy(out)
         else:
             assert st == want
+            msg = lib.nrg_chem_last_error()
+            assert b"KEROMNES: " in msg and b"ZHANG: " in msg, msg
         lib.nrg_mech_free(h)
 
 
