@@ -42,6 +42,7 @@ static const nrg_topo_ops *topology(int i)
   switch (i) {
     case 0: return nrg_topo_ops_keromnes();
     case 1: return nrg_topo_ops_zhang();
+    case 2: return nrg_topo_ops_tereza();
     default: return nullptr;
   }
 }

@@ -99,3 +99,4 @@ struct nrg_topo_ops {
 // nrg_chem.hip build, see the Makefile).
 const nrg_topo_ops *nrg_topo_ops_keromnes();
 const nrg_topo_ops *nrg_topo_ops_zhang();
+const nrg_topo_ops *nrg_topo_ops_tereza();

@@ -26,6 +26,19 @@ from .mechanism import P_ATM, R_GAS_J
 # species order of KEROMNES: H2 O2 N2 O H OH H2O HO2 H2O2
 I_H2, I_O2, I_N2, I_O, I_H, I_OH, I_H2O, I_HO2, I_H2O2 = range(9)
 RADICALS_BURNT = {I_OH: 0.010, I_H: 0.005, I_O: 0.003, I_HO2: 1.0e-5, I_H2O2: 1.0e-6}
+_RADICALS_BURNT_BY_NAME = {"OH": 0.010, "H": 0.005, "O": 0.003, "HO2": 1.0e-5, "H2O2": 1.0e-6}
+
+
+def _index(mech):
+    """Positions of the H2-air species in the mechanism's own species list (a
+    mechanism with another order, such as TEREZA's, gets the same mixtures)."""
+    names = [n.upper() for n in mech.species_names]
+    at = {n: names.index(n) for n in ("H2", "O2", "N2", "H2O") if n in names}
+    missing = {"H2", "O2", "N2", "H2O"} - set(at)
+    if missing:
+        raise ValueError("H2-air states need species %s" % sorted(missing))
+    rad = {names.index(k): v for k, v in _RADICALS_BURNT_BY_NAME.items() if k in names}
+    return at, rad
 
 
 def _torch():
@@ -33,24 +46,30 @@ def _torch():
     return torch
 
 
-def _compose(phi, c, W):
-    """Mole fractions (S, n) blending fresh and burnt gas by progress variable c in [0, 1]."""
+def _compose(phi, c, W, idx=None):
+    """Mole fractions (S, n) blending fresh and burnt gas by progress variable c in [0, 1].
+    idx: species positions from _index (default: KEROMNES order)."""
     torch = _torch()
     n = phi.shape[0]
     S = W.shape[0]
+    if idx is None:
+        at = {"H2": I_H2, "O2": I_O2, "N2": I_N2, "H2O": I_H2O}
+        rad = RADICALS_BURNT
+    else:
+        at, rad = idx
     Xu = torch.zeros((S, n), dtype=torch.float64, device=phi.device)
-    Xu[I_H2] = 2.0 * phi
-    Xu[I_O2] = 1.0
-    Xu[I_N2] = 3.762
+    Xu[at["H2"]] = 2.0 * phi
+    Xu[at["O2"]] = 1.0
+    Xu[at["N2"]] = 3.762
     Xb = torch.zeros_like(Xu)
     lean = phi <= 1.0
-    Xb[I_H2O] = torch.where(lean, 2.0 * phi, torch.full_like(phi, 2.0))
-    Xb[I_O2] = torch.where(lean, 1.0 - phi, torch.zeros_like(phi))
-    Xb[I_H2] = torch.where(lean, torch.zeros_like(phi), 2.0 * phi - 2.0)
-    Xb[I_N2] = 3.762
+    Xb[at["H2O"]] = torch.where(lean, 2.0 * phi, torch.full_like(phi, 2.0))
+    Xb[at["O2"]] = torch.where(lean, 1.0 - phi, torch.zeros_like(phi))
+    Xb[at["H2"]] = torch.where(lean, torch.zeros_like(phi), 2.0 * phi - 2.0)
+    Xb[at["N2"]] = 3.762
     Xu = Xu / Xu.sum(0, keepdim=True)
     Xb = Xb / Xb.sum(0, keepdim=True)
-    for k, v in RADICALS_BURNT.items():
+    for k, v in rad.items():
         Xb[k] = Xb[k] + v
     Xb = Xb / Xb.sum(0, keepdim=True)
     X = (1.0 - c) * Xu + c * Xb
@@ -85,7 +104,7 @@ def flame_front_2d(mech, nx=128, ny=128, seed=1234, device="cpu", noise=0.02, wi
     T = (300.0 + 2300.0 * c) * (1.0 + noise * eta)
     phi = 0.5 + 1.5 * torch.rand(n, generator=g, dtype=torch.float64, device=device)
     p = torch.full((n,), P_ATM, dtype=torch.float64, device=device)
-    X = _compose(phi, c, W)
+    X = _compose(phi, c, W, _index(mech))
     return _finish(T, p, X, W)
 
 
@@ -138,7 +157,7 @@ def hot_kernels_3d(mech, n=(256, 256, 256), seed=2024, device="cpu", n_kernels=N
         newT = sub_T + (float(kt[q]) - sub_T) * prof
         Tv[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = torch.maximum(sub_T, newT)
         cv[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = torch.maximum(sub_c, 0.7 * prof)
-    X = _compose(phi, c, W)
+    X = _compose(phi, c, W, _index(mech))
     return _finish(T, p, X, W)
 
 
@@ -160,7 +179,7 @@ def ignition_mix(mech, n, seed=77, device="cpu", frac_window=0.05):
                       0.5 + 1.5 * torch.rand(n, generator=g, dtype=torch.float64, device=device))
     c = torch.where(burnt, torch.ones_like(T), torch.zeros_like(T))
     p = P_ATM * (0.5 + 4.5 * torch.rand(n, generator=g, dtype=torch.float64, device=device))
-    X = _compose(phi, c, W)
+    X = _compose(phi, c, W, _index(mech))
     return _finish(T, p, X, W)
 
 
@@ -181,5 +200,5 @@ def training_states(mech, n, seed=11, device="cpu", T_range=(300.0, 2800.0), phi
     phi = torch.exp(math.log(phi_range[0]) + (math.log(phi_range[1]) - math.log(phi_range[0])) * u())
     p = P_ATM * (p_atm_range[0] + (p_atm_range[1] - p_atm_range[0]) * u())
     c = torch.where(u() < burnt_fraction, u(), torch.zeros(n, dtype=torch.float64, device=device))
-    X = _compose(phi, c, W)
+    X = _compose(phi, c, W, _index(mech))
     return _finish(T, p, X, W)

@@ -31,6 +31,5 @@ def test_learned_run_shift_drift_and_fallback_retraining():
     for t in rep["retrains"]:
         assert t["pairs"] > 0 and t["val_states"] > 0, t
         assert t["kept"] == (t["val_fallback_after"] < t["val_fallback_before"] and
-                             t["val_max_abs_dY_major_after"] <= max(
-                                 t["val_max_abs_dY_major_before"], 0.5 * rep["bars"]["abs_Y_major"]))
+                             t["val_max_abs_dY_major_after"] <= t["val_max_abs_dY_major_before"])
     assert rep["hbm_in_use_peak_GB"] > rep["field_GB"] > rep["state_GB"] > 0
