@@ -1,4 +1,4 @@
-"""Parity at the 256^3 workload's own scale: 10 consecutive split steps of the
+"""Parity at the 256^3 workloads' own scale.  First: 10 consecutive split steps of the
 seeded 256^3 hot-kernel field (KEROMNES, dt = 1e-7 s) on the GPU, in the
 batched headline set-up (persistent work queue, stiffness order, step graph),
 with a seeded sample of 100,000 cells advanced side by side by the oracle from
@@ -98,3 +98,50 @@ def test_256cube_ten_split_steps_sample_vs_oracle(mech, oracle):
         # a decision flip is a rare event, not a pattern
         assert flip.mean() <= 0.01, (k, flip.mean())
     eng.close()
+
+
+@pytest.mark.parametrize("dt", [1.0e-8, 1.0e-6])
+def test_uneven_stiffness_256cube_sample_vs_oracle(mech, oracle, dt):
+    """The uneven-stiffness workload at its own scale: 256^3 cells, 5 % of them
+    fresh mixture in the 1000-1300 K ignition window, the rest burnt or cold,
+    one chemistry step in the persistent work queue (stiffness order, graph).
+    A seeded 100,000-cell sample matches the oracle to the parity bars in Y
+    (1e-9) after the step; the share of sampled cells whose work counters
+    differ is reported (dt = 1e-6 s is ten HMAX steps at least per cell)."""
+    import torch
+
+    from nrg_amd.solver import ChemistryStep
+    S = mech.species_number
+    n = N ** 3
+    st = inputs.ignition_mix(mech, n, seed=2024, device="cuda:0", frac_window=0.05)
+    eng = ChemistryStep(mech, capi.default_opts(form=capi.NRG_FORM_PERSISTENT,
+                                                cell_order=capi.NRG_ORDER_STIFFNESS, use_graph=1))
+    try:
+        Y = st["Y"].contiguous()
+        out = [torch.zeros(n, dtype=torch.float64, device="cuda:0") for _ in range(2 * S + 1)]
+        cnt = [torch.zeros(n, dtype=torch.int32, device="cuda:0") for _ in range(3)]
+        stats = eng.step_dev(capi.flat_grid(n), dt, st["T"].data_ptr(), st["rho"].data_ptr(),
+                             [Y[s].data_ptr() for s in range(S)], [o.data_ptr() for o in out[:S]],
+                             [o.data_ptr() for o in out[S:2 * S]], out[2 * S].data_ptr(),
+                             counters=[c.data_ptr() for c in cnt])
+        torch.cuda.synchronize()
+        idx = np.sort(np.random.default_rng(11).choice(n, size=SAMPLE, replace=False))
+        it = torch.as_tensor(idx, device="cuda:0")
+        T, rho = st["T"][it].cpu().numpy(), st["rho"][it].cpu().numpy()
+        Ys = Y[:, it].cpu().numpy()
+        o = oracle.chem_cells(dt, T, rho, Ys, nthreads=_threads())
+        Yp = np.stack([out[s][it].cpu().numpy() for s in range(S)])
+        dY = np.abs(Yp - o["Y_prod"]) * dt / rho
+        integ = o["status"] == capi.NRG_CELL_OK
+        ns, nf, nj = (c[it].cpu().numpy() for c in cnt)
+        flip = ((ns != o["nstep"]) | (nf != o["nfe"]) | (nj != o["nje"]))[integ]
+        win = integ & (T >= 1000.0) & (T <= 1300.0)
+        print("[parity-uneven] dt %.0e: %d cells, sample %d (%d in the window): max |dY| %.3e, "
+              "counter flips %d (%.5f), mean NFE in the window %.1f, GPU max NSTEP %d" % (
+                  dt, n, SAMPLE, int(win.sum()), dY.max(), int(flip.sum()),
+                  float(flip.mean()), float(o["nfe"][win].mean()), stats.nstep_max), flush=True)
+        assert dY.max() <= TOL_Y_ABS
+        assert win.sum() > 0.03 * SAMPLE
+        assert flip.mean() <= 0.01
+    finally:
+        eng.close()
