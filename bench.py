@@ -9,8 +9,14 @@ layers, dt = 1e-7 s.  One "step" is one split step: the chemistry step
 the legacy CABARET source application and e->T inversion (nrg_chem_closure_dev).
 Fields stay resident in HBM; the timed region holds exactly --steps split steps.
 
-Prints one JSON line (metric, value in cell-steps/s, roofline of the chemistry
-kernel measured with HIP events, CPU baseline from the oracle on a sample).
+Prints one JSON line: metric, value in cell-steps/s, the roofline of the
+chemistry kernels (HIP events on the library's stream; HBM bytes and lane
+share from the committed rocprofv3 summaries), the CPU baseline (the
+reference's own SLATEC integrator built with flang, on the host cores, over a
+sample of the same state) with an audit of the sample against the oracle, and
+legs run from the same initial state: the learned step and the method without
+the network, the table-approximated mode, the low-Mach coupling kernels, and
+the uneven-stiffness field (persistent queue vs one cell per lane at three dt).
 """
 import argparse
 import json

@@ -353,7 +353,7 @@ void *nrg_chem_stream(nrg_chem_t *h);
 
 /*
  * Name of the reaction topology whose kernels serve the handle ("KEROMNES",
- * "ZHANG", "TEREZA"), or NULL for a NULL handle.  No counterpart in the reference, which
+ * "ZHANG", "TEREZA", "KONNOV", "ACETYLENE"), or NULL for a NULL handle.  No counterpart in the reference, which
  * interprets any mechanism at run time: here the kernels unroll a mechanism's
  * reaction lists at compile time, one build per topology, and nrg_chem_create
  * picks the build whose reaction types and species lists

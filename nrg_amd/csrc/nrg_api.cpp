@@ -43,6 +43,8 @@ static const nrg_topo_ops *topology(int i)
     case 0: return nrg_topo_ops_keromnes();
     case 1: return nrg_topo_ops_zhang();
     case 2: return nrg_topo_ops_tereza();
+    case 3: return nrg_topo_ops_konnov();
+    case 4: return nrg_topo_ops_acetylene();
     default: return nullptr;
   }
 }
@@ -89,7 +91,7 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
 {
   if (!out) return set_error(NRG_CHEM_E_ARG, "out is NULL");
   *out = nullptr;
-  char why[400] = "";       // each build's reason, "NAME: detail; ..."
+  char why[480] = "";       // each build's reason, "NAME: detail; ..."
   for (int i = 0; const nrg_topo_ops *ops = topology(i); ++i) {
     int st = ops->check(mech);
     if (st == NRG_CHEM_E_MECH) {

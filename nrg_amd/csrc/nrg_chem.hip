@@ -1394,12 +1394,12 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 // hipGetSymbolAddress returned (g_noalloc_scratch), never through its name.
 // ===========================================================================
 constexpr size_t NOALLOC_ACC_BYTES = 512;
+#if NRG_TOPO_PRIMARY
 static_assert(ACC_SLOTS * sizeof(unsigned long long) <= NOALLOC_ACC_BYTES, "accumulators");
 static_assert(NOALLOC_ACC_BYTES + (size_t)NRG_NOALLOC_CHUNK_CELLS * (2 * R + NM) * sizeof(double) <=
                   (size_t)NRG_NOALLOC_SCRATCH_BYTES, "rate constants of one chunk");
 static_assert((long long)NRG_NOALLOC_MAX_CELLS <= (long long)MAX_CHUNKS * NRG_NOALLOC_CHUNK_CELLS,
               "one queue head per chunk");
-#if NRG_TOPO_PRIMARY
 // (one build of the library holds the array; the others refuse the no-alloc entry point)
 static __device__ __align__(256) unsigned char scratch[NRG_NOALLOC_SCRATCH_BYTES];
 #endif

@@ -100,3 +100,5 @@ struct nrg_topo_ops {
 const nrg_topo_ops *nrg_topo_ops_keromnes();
 const nrg_topo_ops *nrg_topo_ops_zhang();
 const nrg_topo_ops *nrg_topo_ops_tereza();
+const nrg_topo_ops *nrg_topo_ops_konnov();
+const nrg_topo_ops *nrg_topo_ops_acetylene();

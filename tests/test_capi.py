@@ -106,7 +106,9 @@ def test_mechanism_topology_is_checked(mech):
     ms = capi.MechStruct(m2)
     h = C.c_void_p()
     rc = lib.nrg_chem_create(C.byref(ms.struct), None, C.byref(h))
-    # without a GPU the device check may answer first; with one, the topology check must
-    assert rc in (capi.NRG_CHEM_E_MECH, capi.NRG_CHEM_E_NO_DEVICE)
-    if rc == capi.NRG_CHEM_E_MECH:
-        assert b"chemical_coeffs" in lib.nrg_chem_last_error()
+    # the topology is checked against every compiled build before any device work
+    assert rc == capi.NRG_CHEM_E_MECH and h.value is None
+    msg = lib.nrg_chem_last_error()
+    assert b"KEROMNES: chemical_coeffs" in msg, msg
+    for name in (b"ZHANG: ", b"TEREZA: ", b"KONNOV: ", b"ACETYLENE: "):
+        assert name in msg, msg
