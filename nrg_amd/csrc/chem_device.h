@@ -443,18 +443,41 @@ NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[SA],
     rhs_reaction(r, kl[r * KS], kl[(R + r) * KS], Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0], Z, Zd);
 }
 
+// Species (bit s = species s) on one side of reaction r, third body left out.
+constexpr unsigned side_mask(int r, int side)
+{
+  unsigned m = 0;
+  for (int c = 0; c < 3; ++c) {
+    const int sp = side ? topo::right[r][c] : topo::left[r][c];
+    const int n = side ? topo::nr[r] : topo::nl[r];
+    if (c < n && sp >= 0 && sp < S) m |= 1u << sp;
+  }
+  return m;
+}
+constexpr bool has_reverse(int r)
+{
+  return topo::rtype[r] == 0 || topo::rtype[r] == 1 || topo::rtype[r] == 2 || topo::rtype[r] == 3;
+}
+static_assert(S <= 32, "species masks are 32-bit");
+
 // The part of omega(C, T) of the active species that reactions [R0, R1)
 // contribute, with the rate constants computed one reaction at a time (no
-// array of all k_f, k_r is held).
+// array of all k_f, k_r is held).  A reaction whose left side holds a species
+// the cell does not contain has a zero forward rate, and likewise for the
+// right side and the reverse rate; when both are zero for every lane of the
+// wave, the reaction adds nothing and is skipped (fresh gas, whose radicals
+// are exact zeros, needs a handful of the reactions).
 template <class Mth, int R0 = 0, int R1 = R>
 NRG_DI void explicit_rates(const MechCoef &mc, double Tin, const double (&C)[S], double (&Zd)[SA])
 {
   const double T = dmin(Tin, 10000.0);
   double sv[S], hv[S], Z[SA];
+  unsigned absent = 0;
 #pragma unroll
   for (int c = 0; c < S; ++c) {
     sv[c] = s_molar<Mth>(mc, T, c);
     hv[c] = h_molar(mc, T, c);
+    absent |= (C[c] == 0.0) ? 1u << c : 0u;
   }
 #pragma unroll
   for (int i = 0; i < SA; ++i) {
@@ -463,9 +486,13 @@ NRG_DI void explicit_rates(const MechCoef &mc, double Tin, const double (&C)[S],
   }
 #pragma unroll
   for (int r = R0; r < R1; ++r) {
-    double kf, kr, m;
-    reaction_k<Mth>(mc, T, sv, hv, C, r, kf, kr, m);
-    rhs_reaction(r, kf, kr, m, Z, Zd);
+    const bool fwd = (absent & side_mask(r, 0)) == 0;
+    const bool rev = has_reverse(r) && (absent & side_mask(r, 1)) == 0;
+    if (__any((fwd || rev) ? 1 : 0)) {
+      double kf, kr, m;
+      reaction_k<Mth>(mc, T, sv, hv, C, r, kf, kr, m);
+      rhs_reaction(r, kf, kr, m, Z, Zd);
+    }
   }
 }
 
