@@ -26,6 +26,7 @@ module nrg_chem_binding
   integer(c_int), parameter, public :: NRG_JAC_FD_SLATEC = 0, NRG_JAC_ANALYTIC = 1
   integer(c_int), parameter, public :: NRG_FORM_PLAIN = 0, NRG_FORM_PERSISTENT = 1
   integer(c_int), parameter, public :: NRG_ORDER_NATURAL = 0, NRG_ORDER_STIFFNESS = 1
+  integer(c_int), parameter, public :: NRG_NOALLOC_MAX_CELLS = 48 * 131072
 
   ! struct nrg_chem_mech: the arrays of chemical_properties / thermophysical_properties
   type, bind(C), public :: nrg_chem_mech
@@ -77,6 +78,7 @@ module nrg_chem_binding
   public :: nrg_chem_set_table, nrg_chem_table_step, nrg_chem_table_step_dev
   public :: nrg_low_mach_state_equation_dev, nrg_low_mach_species_dev
   public :: nrg_low_mach_chem_divergence_dev
+  public :: nrg_chem_step_dev_noalloc, nrg_chem_topology, nrg_chem_topology_name
 
   interface
     subroutine nrg_chem_default_opts(opts) bind(C, name="nrg_chem_default_opts")
@@ -135,6 +137,28 @@ module nrg_chem_binding
       type(c_ptr), value :: T, p, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status
       type(nrg_chem_stats), intent(out) :: stats
       integer(c_int) :: status
+    end function
+
+    ! nrg_chem_step_dev without any device allocation (no reference counterpart:
+    ! for callers that cannot allocate device memory; at most
+    ! NRG_NOALLOC_MAX_CELLS inner cells, one call at a time per device)
+    function nrg_chem_step_dev_noalloc(handle, grid, dt, T, p, rho, Y, bc, Y_prod, Y_prod2, &
+                                       E_f_prod, cell_status, stats) &
+        bind(C, name="nrg_chem_step_dev_noalloc") result(status)
+      import :: c_int, c_ptr, c_double, nrg_grid, nrg_chem_stats
+      type(c_ptr), value :: handle
+      type(nrg_grid), intent(in) :: grid
+      real(c_double), value :: dt
+      type(c_ptr), value :: T, p, rho, Y, bc, Y_prod, Y_prod2, E_f_prod, cell_status
+      type(nrg_chem_stats), intent(out) :: stats
+      integer(c_int) :: status
+    end function
+
+    ! name of the reaction-topology build serving the handle (C string)
+    function nrg_chem_topology(handle) bind(C, name="nrg_chem_topology") result(name)
+      import :: c_ptr
+      type(c_ptr), value :: handle
+      type(c_ptr) :: name
     end function
 
     ! read_chemical_kinetics_table (chemical_kinetics_solver.f90:608-648):
@@ -275,6 +299,13 @@ module nrg_chem_binding
   end interface
 
 contains
+
+  ! Name of the kernel build that serves a handle ("KEROMNES", "ZHANG", ...).
+  function nrg_chem_topology_name(handle) result(name)
+    type(c_ptr), intent(in) :: handle
+    character(len=:), allocatable :: name
+    name = c_string(nrg_chem_topology(handle))
+  end function
 
   ! Text of a status code followed by the library's last detailed message.
   function nrg_chem_message(status) result(msg)

@@ -121,6 +121,7 @@ program nrg_chem_fortran_check
   print '(a,i0,a,a)', 'create status ', status, ' ', nrg_chem_message(status)
   if (status == NRG_CHEM_E_NO_DEVICE) stop 3
   if (status /= NRG_CHEM_OK) stop 1
+  print '(a,a)', 'topology ', nrg_chem_topology_name(handle)
   status = nrg_chem_step(handle, grid, dt, c_loc(T), c_null_ptr, c_loc(rho), c_loc(pY), &
                          c_loc(bc), c_loc(pYp), c_loc(pYp2), c_loc(E_f_prod), c_null_ptr, stats)
   print '(a,i0,a,i0,a,i0)', 'step status ', status, ' integrated ', stats%cells_integrated, &

@@ -100,6 +100,8 @@ def test_fortran_binding_step_matches_oracle(mech, oracle, tmp_path):
     r = subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, (r.stdout, r.stderr)
+    # the Fortran caller names the kernel build its mechanism picked
+    assert "topology KEROMNES" in r.stdout, r.stdout
     S = mech.species_number
     status, counts, E, Yp, Yp2 = _read_out(tmp_path / "out.bin", S)
     assert status == 0
