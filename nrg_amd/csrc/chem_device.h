@@ -201,6 +201,31 @@ constexpr bool uses_m(int r)
          topo::rtype[r] == 6 || topo::rtype[r] == 7;
 }
 
+// Cell record: what rates_kernel hands the integrator for one queue position
+// t of a chunk of nrec positions, as pairs of doubles (16 bytes: one lane's
+// share of a direct HBM -> LDS load, global_load_lds_dwordx4).
+//   pairs [0, R)             (k_f(r), k_r(r))
+//   pairs [REC_M, REC_C)     [M] slots 2j, 2j+1
+//   pairs [REC_C, REC_PAIRS) C_s (mol/m^3) at slot s, then the gate flag at
+//                            slot S (1: integrate, 0: gated)
+// In HBM pair p of position t sits at rec[2 (p nrec + t)] (pair-major: a
+// wave's lanes read 1 KiB runs); the integrator keeps the lane's copy in LDS,
+// pair p of lane l at recl[2 (p WAVE + l)].
+constexpr int REC_M = R;
+constexpr int REC_C = REC_M + (NM + 1) / 2;
+constexpr int REC_PAIRS = REC_C + (S + 2) / 2;
+constexpr int REC_FLAG = S;
+// Offset (doubles) of slot j of the pairs starting at pair p0, position 0.
+NRG_DI long long rec_slot(int p0, int j, long long nrec)
+{
+  return 2 * ((long long)(p0 + j / 2) * nrec) + (j & 1);
+}
+// The lane's LDS copy (kl = recl + 2 * lane).
+NRG_DI double rec_kf(const double *kl, int r) { return kl[r * 2 * WAVE]; }
+NRG_DI double rec_kr(const double *kl, int r) { return kl[r * 2 * WAVE + 1]; }
+NRG_DI double rec_m(const double *kl, int j) { return kl[(REC_M + j / 2) * 2 * WAVE + (j & 1)]; }
+NRG_DI double rec_c(const double *kl, int s) { return kl[(REC_C + s / 2) * 2 * WAVE + (s & 1)]; }
+
 // Species that no reaction consumes or produces (N2 in H2-air) enter only
 // through the third-body concentrations, which are fixed per cell.  For them
 // f_s = 0 and no f_i depends on y_s, so under DDRIV3 every correction, every
@@ -302,9 +327,9 @@ NRG_DI double e_molar(const MechCoef &mc, double T, int s)
 }
 
 // ---------------------------------------------------------------------------
-// Rate constants (chemical_kinetics_solver.f90:360-493) of one cell:
-// kout[r*stride] = k_f(r), kout[(R+r)*stride] = k_r(r), and [M] of the
-// reactions whose RHS term uses it, kout[(2R+slot)*stride] (:235-248).
+// Rate constants (chemical_kinetics_solver.f90:360-493) of one cell, written
+// into its cell record (below): k_f(r) and k_r(r) as pair r, and [M] of the
+// reactions whose RHS term uses it (:235-248) in the [M] pairs.
 // ---------------------------------------------------------------------------
 // Reactions [R0, R1) only (their k_f, k_r and [M] slots).
 // k_f, k_r and [M] of reaction r at T (already capped at 1e4 K), from the
@@ -379,8 +404,8 @@ NRG_DI void reaction_k(const MechCoef &mc, double T, const double (&sv)[S], cons
 }
 
 template <class M, int R0 = 0, int R1 = R>
-NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], double *kout,
-                           long long stride)
+NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S], double *rec,
+                           long long nrec)
 {
   const double T = dmin(Tin, 10000.0);
   double sv[S], hv[S];
@@ -393,9 +418,8 @@ NRG_DI void rate_constants(const MechCoef &mc, double Tin, const double (&C)[S],
   for (int r = R0; r < R1; ++r) {
     double kf, kr, m;
     reaction_k<M>(mc, T, sv, hv, C, r, kf, kr, m);
-    if (uses_m(r) && kMSlot.v[r] >= 0) kout[(2 * R + kMSlot.v[r]) * stride] = m;
-    kout[r * stride] = kf;
-    kout[(R + r) * stride] = kr;
+    if (uses_m(r) && kMSlot.v[r] >= 0) rec[rec_slot(REC_M, kMSlot.v[r], nrec)] = m;
+    *reinterpret_cast<double2 *>(rec + rec_slot(r, 0, nrec)) = make_double2(kf, kr);
   }
 }
 
@@ -431,16 +455,15 @@ NRG_DI void rhs_reaction(int r, double kf, double kr, double m, const double (&Z
   }
 }
 
-// kl holds k_f(r) at kl[r*KS] and k_r(r) at kl[(R+r)*KS] (KS = WAVE: the
-// integrator's lane-interleaved LDS copy).
-template <int KS = WAVE>
-NRG_DI void rhs(const double *kl, const double (&Mc)[NM], const double (&Z)[SA], double (&Zd)[SA])
+// Rate constants and [M] from the lane's cell record in LDS (kl).
+NRG_DI void rhs(const double *kl, const double (&Z)[SA], double (&Zd)[SA])
 {
 #pragma unroll
   for (int s = 0; s < SA; ++s) Zd[s] = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r)
-    rhs_reaction(r, kl[r * KS], kl[(R + r) * KS], Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0], Z, Zd);
+    rhs_reaction(r, rec_kf(kl, r), rec_kr(kl, r), kMSlot.v[r] >= 0 ? rec_m(kl, kMSlot.v[r]) : 0.0,
+                 Z, Zd);
 }
 
 // Species (bit s = species s) on one side of reaction r, third body left out.
@@ -1296,8 +1319,7 @@ NRG_DI void advance(Cell &c, double tout, const BdfConst &k, const BdfCoef &mc)
 // into P are plain register moves instead of select chains over all nine
 // columns (measured: 11-18 % less time per step than one shared loop body,
 // despite nine RHS copies).
-NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const BdfCoef &mc, const double *kl,
-                      const double (&Mc)[NM])
+NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const BdfCoef &mc, const double *kl)
 {
   if (mine) {
     c.nje++;
@@ -1334,7 +1356,7 @@ NRG_DI void jac_batch(Cell &c, bool mine, const BdfConst &k, const BdfCoef &mc, 
       const double yp = yj + dy;
 #pragma unroll
       for (int q = 0; q < SA; ++q) c.y[q] = (q == j) ? yp : c.y[q];
-      rhs(kl, Mc, c.y, f1);
+      rhs(kl, c.y, f1);
 #pragma unroll
       for (int q = 0; q < SA; ++q) c.y[q] = (q == j) ? yj : c.y[q];
       const double factor = ((-l0) * c.h) / dy;
@@ -1398,8 +1420,7 @@ constexpr int nu_of(int r, int s)
 // has no derivative), as DDPST forms it for a user Jacobian (MITER = 1).  No
 // extra right-hand sides are evaluated; the step sequence therefore differs
 // from the finite-difference (SLATEC MITER = 2) parity mode.
-NRG_DI void jac_analytic(Cell &c, bool mine, const BdfCoef &mc, const double *kl,
-                         const double (&Mc)[NM])
+NRG_DI void jac_analytic(Cell &c, bool mine, const BdfCoef &mc, const double *kl)
 {
   if (!mine) return;
   c.nje++;
@@ -1411,11 +1432,11 @@ NRG_DI void jac_analytic(Cell &c, bool mine, const BdfCoef &mc, const double *kl
     for (int i = 0; i < SA; ++i) c.P[q][i] = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const double m = Mc[kMSlot.v[r] >= 0 ? kMSlot.v[r] : 0];
+    const double m = kMSlot.v[r] >= 0 ? rec_m(kl, kMSlot.v[r]) : 0.0;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int n = side == 0 ? topo::nl[r] : topo::nr[r];
-      const double k = side == 0 ? kl[r * WAVE] : -kl[(R + r) * WAVE];
+      const double k = side == 0 ? rec_kf(kl, r) : -rec_kr(kl, r);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const int sa = side == 0 ? topo::left[r][a] : topo::right[r][a];

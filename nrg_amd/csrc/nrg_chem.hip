@@ -114,7 +114,7 @@ struct StepArgs {
   unsigned long long *acc;              // [16] work/status accumulators
   unsigned long long *queue;            // next unclaimed cell of the chunk (persistent form)
   int jac_pct;                          // run a Jacobian batch when >= this % of pending lanes want one
-  const double *rates;                  // [2R+NM][nchunk] k_f, k_r, [M] of the chunk's cells
+  const double *rates;                  // cell records of the chunk's positions (REC_PAIRS pairs each)
   double *rates_out;                    // same buffer, written by rates_kernel
   long long base, nchunk;               // this launch covers queue positions [base, base+nchunk)
   const int32_t *list;                  // NULL: position t is inner cell t; else inner cell list[t]
@@ -177,15 +177,43 @@ __global__ void reset_kernel(unsigned long long *acc, int n)
   if ((int)threadIdx.x < n) acc[threadIdx.x] = 0ull;
 }
 
-// Load one cell and set it up (chemical_kinetics_solver.f90:228-265).
-// Returns false for a gated cell, whose outputs are written here (:341-346).
-NRG_DI bool start_cell(const StepArgs &a, const BdfCoef &mc, long long t, long long q,
-                       long long &off, Cell &c, double *kl, double (&Mc)[NM])
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(3))) double lds_double;
+
+// The integrator keeps the whole cell record in LDS when four single-wave
+// blocks still fit a CU's 160 KiB (every mechanism up to ~37 record pairs);
+// a larger mechanism keeps the rate constants and [M] there and reads the
+// initial concentrations and the gate flag from HBM.
+constexpr size_t LDS_PER_CU = 160 * 1024;
+constexpr bool REC_C_IN_LDS = 4 * ((size_t)REC_PAIRS * 16 * WAVE + sizeof(BdfCoef)) <= LDS_PER_CU;
+constexpr int REC_LDS_PAIRS = REC_C_IN_LDS ? REC_PAIRS : REC_C;
+
+// Load one cell and set it up (chemical_kinetics_solver.f90:228-265).  The
+// cell record of position t (rates_kernel) goes from HBM straight into the
+// lane's LDS slots (recl: the block's record array, kl: this lane's view of
+// it): one round trip, no registers in between.  Returns false for a gated
+// cell, whose outputs are written here (:341-346).
+NRG_DI bool start_cell(const StepArgs &a, long long t, long long q, long long &off, Cell &c,
+                       lds_double *recl, const double *kl)
 {
+  const double *src = a.rates + 2 * t;
+#pragma unroll
+  for (int p = 0; p < REC_LDS_PAIRS; ++p)
+    __builtin_amdgcn_global_load_lds((const void *)(src + 2 * (long long)p * a.nchunk),
+                                     (lds_void *)(recl + 2 * p * WAVE), 16, 0, 0);   // M0: LDS address
+  double cs[S + 1];
+  if (!REC_C_IN_LDS) {
+#pragma unroll
+    for (int s = 0; s <= S; ++s) cs[s] = src[rec_slot(REC_C, s, a.nchunk)];
+  }
   off = cell_offset(a.g, q);
-  const double T = a.T[off];
-  const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
-  if (!(T >= a.k.t_gate && bcv == 0)) {
+  // a direct load's data is in LDS once vmcnt has counted it
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  if (REC_C_IN_LDS) {
+#pragma unroll
+    for (int s = 0; s <= S; ++s) cs[s] = rec_c(kl, s);
+  }
+  if (cs[REC_FLAG] == 0.0) {
 #pragma unroll
     for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
     a.Efp[off] = 0.0;
@@ -198,27 +226,21 @@ NRG_DI bool start_cell(const StepArgs &a, const BdfCoef &mc, long long t, long l
     if (a.hist) a.hist[q] = 0;
     return false;
   }
-  const double rho = a.rho[off];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const double cs = (a.Y[s][off] * rho) / mc.W[s];
     if (kSp.act[s] >= 0)
-      c.y[kSp.act[s]] = cs;
+      c.y[kSp.act[s]] = cs[s];
     else
-      c.yi[kSp.islot[s]] = cs;
+      c.yi[kSp.islot[s]] = cs[s];
   }
-  const double *rk = a.rates + t;
-#pragma unroll
-  for (int r = 0; r < 2 * R; ++r) kl[r * WAVE] = rk[r * a.nchunk];
-#pragma unroll
-  for (int i = 0; i < NM; ++i) Mc[i] = rk[(2 * R + i) * a.nchunk];
   bdf_init(c, a.dt, a.k);
   return true;
 }
 
-// Rate constants and [M] of every integrated cell of the chunk
-// (chemical_kinetics_solver.f90:235-252, 360-493), kept apart from the
-// integrator so its ~80 transcendentals run coherently.  A block covers
+// Cell records of the chunk's positions: the gate (:224-227), and for every
+// integrated cell its concentrations (:228-233), rate constants and [M]
+// (:235-252, 360-493), kept apart from the integrator so its ~80
+// transcendentals run coherently.  A block covers
 // RATES_CELLS cells RATES_GROUPS times: the waves of group g compute
 // reactions [g R / G, (g+1) R / G) (a wave-uniform split, so each wave's code
 // and registers cover a part of the mechanism: occupancy 4 instead of 1-2).
@@ -238,14 +260,26 @@ __global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
   const long long off = cell_offset(a.g, q);
   const double T = a.T[off];
   const int8_t bcv = a.bc ? a.bc[off] : (int8_t)0;
-  if (!(T >= a.k.t_gate && bcv == 0)) return;
+  double *out = a.rates_out + 2 * t;
+  if (!(T >= a.k.t_gate && bcv == 0)) {
+    if (grp == 0)   // the gate flag's pair
+      *reinterpret_cast<double2 *>(out + rec_slot(REC_C, REC_FLAG & ~1, a.nchunk)) = make_double2(0.0, 0.0);
+    return;
+  }
   const double rho = a.rho[off];
   double C[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) / mc.W[s];
+  if (grp == 0) {
+#pragma unroll
+    for (int j = 0; j <= REC_FLAG; j += 2) {
+      const double lo = j < S ? C[j] : 1.0;
+      const double hi = j + 1 < S ? C[j + 1] : (j + 1 == REC_FLAG ? 1.0 : 0.0);
+      *reinterpret_cast<double2 *>(out + rec_slot(REC_C, j, a.nchunk)) = make_double2(lo, hi);
+    }
+  }
   constexpr int G = RATES_GROUPS;
   static_assert(G == 1 || G == 2 || G == 4, "RATES_GROUPS must be 1, 2 or 4");
-  double *out = a.rates_out + t;
   if (G == 1 || grp == 0)
     rate_constants<InlineMath, 0, R / G>(mc, T, C, out, a.nchunk);
   else if (G == 2 || grp == 1)
@@ -259,13 +293,14 @@ __global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
 // Output assembly (:318-340) once the cell's DDRIV3 has returned; returns the
 // cell's NRG_CELL_* status.
 NRG_DI int finish_cell(const StepArgs &a, const BdfCoef &mc, long long q, long long off,
-                       const Cell &c)
+                       const Cell &c, const double *kl)
 {
-  const double rho = a.rho[off];
+  const double rho = REC_C_IN_LDS ? 0.0 : a.rho[off];
   double ef = 0.0;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    const double cin = (a.Y[s][off] * rho) / mc.W[s];
+    // the initial concentration as rates_kernel recorded it (the same expression)
+    const double cin = REC_C_IN_LDS ? rec_c(kl, s) : (a.Y[s][off] * rho) / mc.W[s];
     const double ys = (kSp.act[s] >= 0) ? c.y[kSp.act[s]] : c.yi[kSp.islot[s]];
     const double dc = ys - cin;
     a.Yp[s][off] = (dc * mc.W[s]) / a.dt;
@@ -347,14 +382,12 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(NRG_B
 chem_bdf_kernel(StepArgs a)
 {
   __shared__ BdfCoef mc;
-  __shared__ double krate[2 * R * WAVE];
+  __shared__ __align__(16) double recl[2 * REC_LDS_PAIRS * WAVE];   // the lanes' cell records
   stage_bdf(mc, a.mech);
   const int lane = threadIdx.x;
-  double *kl = krate + lane;
+  const double *kl = recl + 2 * lane;
+  lds_double *recl_lds = (lds_double *)recl;   // the array's LDS address (direct loads' M0)
   Cell c;
-  double Mc[NM];
-#pragma unroll
-  for (int i = 0; i < NM; ++i) Mc[i] = 0.0;
   c.req = REQ_DONE;
   long long q = -1, off = 0;
   bool have = false, exhausted = false;
@@ -368,7 +401,7 @@ chem_bdf_kernel(StepArgs a)
     const long long t = (long long)blockIdx.x * WAVE + lane;
     if (t < a.nchunk) {
       q = cell_of(a, t);
-      have = start_cell(a, mc, t, q, off, c, kl, Mc);
+      have = start_cell(a, t, q, off, c, recl_lds, kl);
       gated = !have;
     }
   }
@@ -391,7 +424,7 @@ chem_bdf_kernel(StepArgs a)
             exhausted = true;
           } else {
             q = cell_of(a, t);
-            have = start_cell(a, mc, t, q, off, c, kl, Mc);
+            have = start_cell(a, t, q, off, c, recl_lds, kl);
             if (!have) tl.cnt[NRG_CELL_GATED]++;
           }
         }
@@ -406,18 +439,18 @@ chem_bdf_kernel(StepArgs a)
     const bool run_jac = nj > 0 && (nf == 0 || nj * 100 >= a.jac_pct * (nj + nf));
     if (run_jac) {
       if (ANALYTIC)
-        jac_analytic(c, wantj, mc, kl, Mc);
+        jac_analytic(c, wantj, mc, kl);
       else
-        jac_batch(c, wantj, a.k, mc, kl, Mc);
+        jac_batch(c, wantj, a.k, mc, kl);
     }
     else if (wantf)
-      rhs(kl, Mc, c.y, c.save2);
+      rhs(kl, c.y, c.save2);
     if (have && (c.req == REQ_NONE || (run_jac ? wantj : wantf))) {
       c.req = REQ_NONE;
       advance(c, a.dt, a.k, mc);
     }
     if (have && c.req == REQ_DONE) {
-      status = finish_cell(a, mc, q, off, c);
+      status = finish_cell(a, mc, q, off, c, kl);
       if (PERSISTENT) tl.integrated(c, status);
       have = false;
     }
@@ -1388,7 +1421,7 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 
 // ===========================================================================
 // Scratch of nrg_chem_step_dev_noalloc: the work accumulators and queue heads
-// (ACC_SLOTS words, padded to 512 bytes), then the rate constants of one chunk
+// (ACC_SLOTS words, padded to 512 bytes), then the cell records of one chunk
 // of NRG_NOALLOC_CHUNK_CELLS cells.  One copy per device, held until the
 // process ends.  Host code reaches it only through the per-device address
 // hipGetSymbolAddress returned (g_noalloc_scratch), never through its name.
@@ -1396,8 +1429,8 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
 constexpr size_t NOALLOC_ACC_BYTES = 512;
 #if NRG_TOPO_PRIMARY
 static_assert(ACC_SLOTS * sizeof(unsigned long long) <= NOALLOC_ACC_BYTES, "accumulators");
-static_assert(NOALLOC_ACC_BYTES + (size_t)NRG_NOALLOC_CHUNK_CELLS * (2 * R + NM) * sizeof(double) <=
-                  (size_t)NRG_NOALLOC_SCRATCH_BYTES, "rate constants of one chunk");
+static_assert(NOALLOC_ACC_BYTES + (size_t)NRG_NOALLOC_CHUNK_CELLS * REC_PAIRS * 2 * sizeof(double) <=
+                  (size_t)NRG_NOALLOC_SCRATCH_BYTES, "cell records of one chunk");
 static_assert((long long)NRG_NOALLOC_MAX_CELLS <= (long long)MAX_CHUNKS * NRG_NOALLOC_CHUNK_CELLS,
               "one queue head per chunk");
 // (one build of the library holds the array; the others refuse the no-alloc entry point)
@@ -1431,7 +1464,7 @@ struct nrg_chem_handle {
   int persistent_blocks = 0;               // resident one-wave blocks of the persistent form
   // staging buffers for the host-pointer entry point
   std::vector<void *> stage;
-  // rate-constant buffer of one chunk: [2R+NM][chunk] doubles
+  // cell records of one chunk: REC_PAIRS pairs of doubles per position
   double *rates = nullptr;
   long long rates_cells = 0;
   long long chunk_max = 1ll << 23;
@@ -1839,7 +1872,7 @@ static int prepare_rates(nrg_chem_t *h, long long npos)
     if (h->rates) HIP_TRY(hipFree(h->rates));
     h->rates = nullptr;
     h->rates_cells = 0;
-    HIP_TRY(hipMalloc(&h->rates, (size_t)chunk * (2 * R + NM) * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->rates, (size_t)chunk * REC_PAIRS * 2 * sizeof(double)));
     h->rates_cells = chunk;
   }
   return NRG_CHEM_OK;
