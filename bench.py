@@ -36,8 +36,9 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
+    # defaults: the configuration the committed rocprofv3 summaries were taken on
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256, help="cells per dimension")
     ap.add_argument("--dt", type=float, default=1.0e-7)
     ap.add_argument("--seed", type=int, default=2024)

@@ -270,14 +270,6 @@ __global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
   double C[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) / mc.W[s];
-  if (grp == 0) {
-#pragma unroll
-    for (int j = 0; j <= REC_FLAG; j += 2) {
-      const double lo = j < S ? C[j] : 1.0;
-      const double hi = j + 1 < S ? C[j + 1] : (j + 1 == REC_FLAG ? 1.0 : 0.0);
-      *reinterpret_cast<double2 *>(out + rec_slot(REC_C, j, a.nchunk)) = make_double2(lo, hi);
-    }
-  }
   constexpr int G = RATES_GROUPS;
   static_assert(G == 1 || G == 2 || G == 4, "RATES_GROUPS must be 1, 2 or 4");
   if (G == 1 || grp == 0)
@@ -288,6 +280,16 @@ __global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
     rate_constants<InlineMath, 2 * R / G, 3 * R / G>(mc, T, C, out, a.nchunk);
   else
     rate_constants<InlineMath, 3 * R / G, R>(mc, T, C, out, a.nchunk);
+  // the concentrations and the gate flag, written last (stored ahead of the
+  // rate constants, they doubled the kernel's registers: occupancy 4 -> 2)
+  if (grp == 0) {
+#pragma unroll
+    for (int j = 0; j <= REC_FLAG; j += 2) {
+      const double lo = j < S ? C[j] : 1.0;
+      const double hi = j + 1 < S ? C[j + 1] : (j + 1 == REC_FLAG ? 1.0 : 0.0);
+      *reinterpret_cast<double2 *>(out + rec_slot(REC_C, j, a.nchunk)) = make_double2(lo, hi);
+    }
+  }
 }
 
 // Output assembly (:318-340) once the cell's DDRIV3 has returned; returns the
