@@ -63,6 +63,9 @@ def parse():
                     help="batched-Jacobian threshold in percent of pending lanes (0: default)")
     ap.add_argument("--gather-pct", type=int, default=0,
                     help="error-test gathering threshold in percent of stepping lanes (same results)")
+    ap.add_argument("--refill-idle", type=int, default=0,
+                    help="persistent form: idle lanes of 64 that trigger a refill (0: default 63; "
+                         "same results)")
     ap.add_argument("--surrogate", default="auto",
                     help="trained surrogate (.npz) timed beside the FP64 integrator from the same "
                          "initial state; 'auto': nrg_amd/data/surrogate_keromnes.npz if present; "
@@ -102,11 +105,13 @@ def effective_setup(args, tuning):
     from nrg_amd import capi
     if tuning is None:
         return {"form": args.form, "order": args.order, "jac_batch_pct": args.jac_pct or 50,
-                "errtest_gather_pct": args.gather_pct, "chunk_log2": args.chunk_log2 or 23}
+                "errtest_gather_pct": args.gather_pct, "refill_min_idle": args.refill_idle or 63,
+                "chunk_log2": args.chunk_log2 or 23}
     b = tuning["best"]
     return {"form": "persistent" if b["form"] == capi.NRG_FORM_PERSISTENT else "plain",
             "order": "stiffness" if b["cell_order"] == capi.NRG_ORDER_STIFFNESS else "natural",
             "jac_batch_pct": b["jac_batch_pct"], "errtest_gather_pct": b["errtest_gather_pct"],
+            "refill_min_idle": b["refill_min_idle"],
             "chunk_log2": int(b["chunk_cells"]).bit_length() - 1}
 
 
@@ -149,7 +154,8 @@ def main():
                                                               cell_order=order, jacobian=jac,
                                                               use_graph=args.graph,
                                                               chunk_log2=args.chunk_log2,
-                                                              errtest_gather_pct=args.gather_pct))
+                                                              errtest_gather_pct=args.gather_pct,
+                                                              refill_min_idle=args.refill_idle))
     E_f = dm.fields["full_energy"]
     solver.engine.energy_from_temperature_dev(g, dm.fields["temperature"].ptr(), Yf.ptrs(),
                                               E_f.ptr())
@@ -581,7 +587,8 @@ def uneven_leg(args, mech, tuning, dts=(1e-8, 1e-7, 1e-6), reps=2):
                                use_graph=1, jac_batch_pct=b.get("jac_batch_pct", 0),
                                waves_per_cu=b.get("waves_per_cu", 0),
                                chunk_log2=(int(b["chunk_cells"]).bit_length() - 1) if b else 0,
-                               errtest_gather_pct=b.get("errtest_gather_pct", 0))
+                               errtest_gather_pct=b.get("errtest_gather_pct", 0),
+                               refill_min_idle=b.get("refill_min_idle", 0))
     plain = capi.default_opts(form=capi.NRG_FORM_PLAIN, cell_order=capi.NRG_ORDER_NATURAL,
                               use_graph=0, chunk_log2=(n - 1).bit_length() if n > 1 else 16)
     res = {"cells": n, "window_frac": 0.05, "reps": reps, "per_dt": []}

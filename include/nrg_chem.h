@@ -140,7 +140,9 @@ typedef struct nrg_chem_opts {
                                   error test until this percentage of the wave's lanes that
                                   are stepping is there (100: the wave's cells take their
                                   steps' error tests together; 0: no waiting).  Same results. */
-  int32_t reserved;
+  int32_t refill_min_idle;     /* persistent form: a wave hands new cells to its idle lanes once
+                                  this many of its 64 lanes are idle, or all are (1..64; 0 =
+                                  default 63: the wave's cells start together).  Same results. */
 } nrg_chem_opts;
 
 /*
@@ -287,8 +289,8 @@ int nrg_energy_from_temperature_dev(nrg_chem_t *h, const nrg_grid *grid,
 /*
  * Autotuner of the batched form.  A cell is one lane and a workgroup one
  * wave (64 cells), so what is tuned is how the queue feeds the waves: launch
- * form, cell order, the batched-Jacobian threshold, resident waves per CU and
- * the queue chunk size.  Starting from the handle's current set-up it tries
+ * form, cell order, the error-test gathering, the batched-Jacobian threshold,
+ * the refill threshold, resident waves per CU and the queue chunk size.  Starting from the handle's current set-up it tries
  * one change at a time (greedy), each as a full chemistry step on the given
  * fields timed with HIP events, keeps the fastest and leaves it in the
  * handle.  Every trial writes the step's outputs (identical for every set-up).
@@ -299,7 +301,7 @@ typedef struct nrg_chem_tune_trial {
   int32_t form, cell_order, jac_batch_pct, waves_per_cu;
   int64_t chunk_cells;
   double kernel_ms;
-  int32_t errtest_gather_pct, reserved;
+  int32_t errtest_gather_pct, refill_min_idle;
 } nrg_chem_tune_trial;
 
 typedef struct nrg_chem_tuning {

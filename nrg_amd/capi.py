@@ -89,7 +89,7 @@ class NrgChemOpts(C.Structure):
         ("use_graph", C.c_int32),
         ("chunk_log2", C.c_int32),
         ("errtest_gather_pct", C.c_int32),
-        ("reserved", C.c_int32),
+        ("refill_min_idle", C.c_int32),
     ]
 
 
@@ -168,7 +168,7 @@ class NrgSurrogateStats(C.Structure):
 class NrgChemTuneTrial(C.Structure):
     _fields_ = [("form", C.c_int32), ("cell_order", C.c_int32), ("jac_batch_pct", C.c_int32),
                 ("waves_per_cu", C.c_int32), ("chunk_cells", C.c_int64), ("kernel_ms", C.c_double),
-                ("errtest_gather_pct", C.c_int32), ("reserved", C.c_int32)]
+                ("errtest_gather_pct", C.c_int32), ("refill_min_idle", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -200,6 +200,7 @@ def default_opts(**kw):
     o.use_graph = 0
     o.chunk_log2 = 0
     o.errtest_gather_pct = 0
+    o.refill_min_idle = 0
     for k, v in kw.items():
         setattr(o, k, v)
     return o

@@ -79,6 +79,7 @@ struct BdfConst {
   double t_gate;
   int mxstep, maxord;
   int gather_pct;
+  int refill_idle;               // persistent form: idle lanes (of 64) that trigger a refill
 };
 
 NRG_DI double dmax(double a, double b) { return (a >= b) ? a : b; }

@@ -368,13 +368,13 @@ struct Tally {
 
 // One chemistry step (solve_chemical_kinetics, :224-346), one cell per lane.
 // PERSISTENT = false: lane i of block b owns inner cell 64*b + i (the plain
-//   launch).  PERSISTENT = true: a resident grid; once REFILL_MIN_IDLE lanes
+//   launch).  PERSISTENT = true: a resident grid; once k.refill_idle lanes
 //   of a wave have finished (or all have), they take the next unclaimed cells
 //   from a.queue (one atomic per wave per refill).
 // Each iteration either evaluates f for the lanes that asked, or -- when at
 // least jac_pct % of the pending lanes wait for a Jacobian, or nothing else is
 // pending -- runs one batched FD Jacobian + LU for the waiting lanes.
-constexpr int REFILL_MIN_IDLE = 48;    // persistent form: idle lanes that trigger a refill
+constexpr int REFILL_MIN_IDLE_DEFAULT = 63;   // measured: 48 -> 41.9, 56 -> 40.5, 63 -> 40.0 ms per 256^3 step
 
 #ifndef NRG_BDF_WAVES_PER_EU
 #define NRG_BDF_WAVES_PER_EU 1   // resident waves per SIMD the register budget is sized for
@@ -412,7 +412,7 @@ chem_bdf_kernel(StepArgs a)
       // refill in batches: new cells start together, so the wave's lanes stay
       // near the same stage (refilling every pass measured 30 % slower)
       bool need = !have && !exhausted;
-      if (wave_any(have) && __popcll(__ballot(need ? 1 : 0)) < REFILL_MIN_IDLE) need = false;
+      if (wave_any(have) && __popcll(__ballot(need ? 1 : 0)) < a.k.refill_idle) need = false;
       while (wave_any(need)) {
         const unsigned long long mask = __ballot(need ? 1 : 0);
         const int leader = __ffsll((long long)mask) - 1;
@@ -1608,6 +1608,7 @@ BdfConst make_const(const nrg_chem_opts &o)
   k.mxstep = o.mxstep;
   k.maxord = std::min(o.mxord, MAXORD);
   k.gather_pct = o.errtest_gather_pct;
+  k.refill_idle = o.refill_min_idle > 0 ? o.refill_min_idle : REFILL_MIN_IDLE_DEFAULT;
   return k;
 }
 
@@ -1723,6 +1724,8 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
     return set_error(NRG_CHEM_E_ARG, "invalid integrator options");
   if (o.errtest_gather_pct < 0 || o.errtest_gather_pct > 100)
     return set_error(NRG_CHEM_E_ARG, "errtest_gather_pct must be in 0..100");
+  if (o.refill_min_idle < 0 || o.refill_min_idle > WAVE)
+    return set_error(NRG_CHEM_E_ARG, "refill_min_idle must be in 0..64");
   if (o.jacobian != NRG_JAC_FD_SLATEC && o.jacobian != NRG_JAC_ANALYTIC)
     return set_error(NRG_CHEM_E_ARG, "jacobian mode %d not available", o.jacobian);
   if (o.form != NRG_FORM_PLAIN && o.form != NRG_FORM_PERSISTENT)
@@ -2537,10 +2540,11 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
   struct Cfg {
     int form, order, jac, wpc;
     long long chunk;
-    int gather;
+    int gather, refill;
   };
   auto apply = [&](const Cfg &c) {
     h->k.gather_pct = c.gather;
+    h->k.refill_idle = c.refill;
     h->form = c.form;
     h->order = c.order;
     h->jac_pct = c.jac;
@@ -2558,6 +2562,7 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
       t.form = c.form; t.cell_order = c.order; t.jac_batch_pct = c.jac;
       t.waves_per_cu = c.wpc > 0 ? c.wpc : occ; t.chunk_cells = c.chunk; t.kernel_ms = ms;
       t.errtest_gather_pct = c.gather;
+      t.refill_min_idle = c.refill;
     }
     return st;
   };
@@ -2570,7 +2575,8 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
     ~Restore() { h->use_graph = g; }
   } restore{h, graph};
   const int wpc0 = h->persistent_blocks / std::max(prop.multiProcessorCount, 1);
-  Cfg best{h->form, h->order, h->jac_pct, wpc0 == occ ? 0 : wpc0, h->chunk_max, h->k.gather_pct};
+  Cfg best{h->form, h->order, h->jac_pct, wpc0 == occ ? 0 : wpc0, h->chunk_max, h->k.gather_pct,
+           h->k.refill_idle};
   double best_ms = 0.0;
   int st;
   // the first run sets up the stiffness history that the ordered trials use
@@ -2611,6 +2617,13 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
       if ((st = consider(c))) return st;
     }
   if (best.form == NRG_FORM_PERSISTENT)
+    for (int refill : {48, REFILL_MIN_IDLE_DEFAULT})
+      if (refill != best.refill) {
+        Cfg c = best;
+        c.refill = refill;
+        if ((st = consider(c))) return st;
+      }
+  if (best.form == NRG_FORM_PERSISTENT)
     for (int w : {2 * occ, 4 * occ}) {
       Cfg c = best;
       c.wpc = w;
@@ -2630,6 +2643,7 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
   out->best.chunk_cells = best.chunk;
   out->best.kernel_ms = best_ms;
   out->best.errtest_gather_pct = best.gather;
+  out->best.refill_min_idle = best.refill;
   return NRG_CHEM_OK;
 }
 

@@ -40,7 +40,7 @@ module nrg_chem_binding
     real(c_double) :: eps, ewt, hmax, t_gate
     integer(c_int32_t) :: mxstep, mxord, jacobian, device, form, jac_batch_pct, waves_per_cu
     integer(c_int32_t) :: cell_order, use_graph, chunk_log2
-    integer(c_int32_t) :: errtest_gather_pct, reserved
+    integer(c_int32_t) :: errtest_gather_pct, refill_min_idle
   end type
 
   type, bind(C), public :: nrg_grid
@@ -61,7 +61,7 @@ module nrg_chem_binding
     integer(c_int32_t) :: form, cell_order, jac_batch_pct, waves_per_cu
     integer(c_int64_t) :: chunk_cells
     real(c_double) :: kernel_ms
-    integer(c_int32_t) :: errtest_gather_pct, reserved
+    integer(c_int32_t) :: errtest_gather_pct, refill_min_idle
   end type
 
   type, bind(C), public :: nrg_chem_tuning

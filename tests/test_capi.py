@@ -90,6 +90,11 @@ def test_errors_are_reported_not_fatal(mech):
         assert lib.nrg_chem_create(C.byref(ms.struct), C.byref(bad), C.byref(h)) == capi.NRG_CHEM_E_ARG
         assert b"errtest_gather_pct" in lib.nrg_chem_last_error()
         assert h.value is None
+    for r in (-1, 65):
+        bad = capi.default_opts(refill_min_idle=r)
+        assert lib.nrg_chem_create(C.byref(ms.struct), C.byref(bad), C.byref(h)) == capi.NRG_CHEM_E_ARG
+        assert b"refill_min_idle" in lib.nrg_chem_last_error()
+        assert h.value is None
     # a NULL handle is an argument error, not a crash
     st = capi.NrgChemStats()
     rc = lib.nrg_chem_step_dev(None, None, 1e-7, *([None] * 9), C.byref(st))

@@ -221,6 +221,29 @@ def test_errtest_gather_is_bitwise_identical(mech, engine, form, gather):
         eng.close()
 
 
+@pytest.mark.parametrize("refill", [1, 48, 64])
+def test_refill_threshold_is_bitwise_identical(mech, engine, refill):
+    """refill_min_idle only decides when a persistent wave hands new cells to
+    its idle lanes (1: as soon as one is idle, 64: once all are): every cell
+    takes the same steps, so outputs and work counters equal the plain
+    one-cell-per-lane step bit for bit."""
+    st = _to_cuda(inputs.ignition_mix(mech, 20000, seed=43, frac_window=0.3))
+    from nrg_amd.solver import ChemistryStep
+    eng = ChemistryStep(mech, capi.default_opts(form=capi.NRG_FORM_PERSISTENT,
+                                                cell_order=capi.NRG_ORDER_STIFFNESS,
+                                                refill_min_idle=refill))
+    try:
+        for dt in (1e-7, 1e-6):
+            a = gpu_flat_step(engine, st, dt)
+            b = gpu_flat_step(eng, st, dt)
+            _report("refill_threshold", refill=refill, dt=dt, nfe=b["stats"].nfe_sum)
+            for k in ("Y_prod", "Y_prod2", "E_f_prod", "status", "nstep", "nfe", "nje"):
+                assert np.array_equal(a[k], b[k]), (dt, k)
+            assert a["stats"].ncorr_sum == b["stats"].ncorr_sum
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("order", [capi.NRG_ORDER_NATURAL, capi.NRG_ORDER_STIFFNESS])
 def test_graph_replay_is_bitwise_identical(mech, engine, order):
     """use_graph = 1: the persistent step is captured once per argument set and
