@@ -1789,23 +1789,22 @@ int nrg_chem_create(const nrg_chem_mech *mech, const nrg_chem_opts *opts, nrg_ch
 void nrg_chem_destroy(nrg_chem_t *h)
 {
   if (!h) return;
-  hipSetDevice(h->device);
-  if (h->stream) hipStreamSynchronize(h->stream);
+  // teardown has no caller to report to: failures here are ignored
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (void *p : h->stage)
-    if (p) hipFree(p);
-  if (h->dev_mech) hipFree(h->dev_mech);
-  if (h->dev_acc) hipFree(h->dev_acc);
-  if (h->rates) hipFree(h->rates);
-  if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
-  if (h->ev0) hipEventDestroy(h->ev0);
-  if (h->ev1) hipEventDestroy(h->ev1);
-  if (h->ev2) hipEventDestroy(h->ev2);
-  if (h->ev3) hipEventDestroy(h->ev3);
+    if (p) (void)hipFree(p);
+  if (h->dev_mech) (void)hipFree(h->dev_mech);
+  if (h->dev_acc) (void)hipFree(h->dev_acc);
+  if (h->rates) (void)hipFree(h->rates);
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  for (hipEvent_t e : {h->ev0, h->ev1, h->ev2, h->ev3})
+    if (e) (void)hipEventDestroy(e);
   for (void *p : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_rate_bad,
                   (void *)h->sur_cand, (void *)h->sur_acc,
                   (void *)h->order_list, (void *)h->hist, (void *)h->bins, (void *)h->table})
-    if (p) hipFree(p);
-  if (h->stream) hipStreamDestroy(h->stream);
+    if (p) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -2164,8 +2163,13 @@ static int step_host(nrg_chem_t *h, bool table, const nrg_grid *grid, double dt,
   // staging layout: T, rho, E_f_prod, Y[S], Y_prod[S], Y_prod2[S] (doubles), bc (int8), status (int32)
   const int nd = 3 + 3 * S;
   if (h->stage_cells < (size_t)n) {
-    for (void *q : h->stage)
-      if (q) hipFree(q);
+    h->stage_cells = 0;
+    for (void *&q : h->stage)
+      if (q) {
+        void *old = q;
+        q = nullptr;   // never freed twice, even when this free fails
+        HIP_TRY(hipFree(old));
+      }
     h->stage.assign(nd + 2, nullptr);
     for (int i = 0; i < nd; ++i) HIP_TRY(hipMalloc(&h->stage[i], n * sizeof(double)));
     HIP_TRY(hipMalloc(&h->stage[nd], n * sizeof(int8_t)));
