@@ -135,7 +135,8 @@ typedef struct nrg_chem_opts {
   int32_t use_graph;           /* 1: capture the step's launches in a HIP graph and replay it
                                   while the arguments stay the same (same results)             */
   int32_t chunk_log2;          /* queue chunk = 2^chunk_log2 cells (16..26; 0 = 23): cells per
-                                  rates buffer and per integrator launch                       */
+                                  integrator launch and per cell-record buffer (496 B per cell
+                                  for KEROMNES: 4.2 GB at 2^23)                                */
   int32_t errtest_gather_pct;  /* 0..100: a lane whose corrector has converged waits at the
                                   error test until this percentage of the wave's lanes that
                                   are stepping is there (100: the wave's cells take their
