@@ -15,8 +15,11 @@ share from the committed rocprofv3 summaries), the CPU baseline (the
 reference's own SLATEC integrator built with flang, on the host cores, over a
 sample of the same state) with an audit of the sample against the oracle, and
 legs run from the same initial state: the learned step and the method without
-the network, the table-approximated mode, the low-Mach coupling kernels, and
-the uneven-stiffness field (persistent queue vs one cell per lane at three dt).
+the network (their FP64 fallback integrating at eps 1e-6 with the analytic
+Jacobian, on a handle of its own, and again at the reference's eps 1e-9 with
+the FD Jacobian), the table-approximated mode, the low-Mach coupling kernels,
+and the uneven-stiffness field (persistent queue vs one cell per lane at
+three dt).
 """
 import argparse
 import json
@@ -71,6 +74,13 @@ def parse():
                          "initial state; 'auto': nrg_amd/data/surrogate_keromnes.npz if present; "
                          "'none': skip")
     ap.add_argument("--gate-energy-tol", type=float, default=None, help="K (default: trained file's)")
+    ap.add_argument("--fallback-eps", type=float, default=1e-6,
+                    help="learned-step legs: integrator eps of the FP64 fallback, on a handle of "
+                         "its own (0: the reference's 1e-9 on the FP64 leg's handle)")
+    ap.add_argument("--fallback-jacobian", choices=["fd", "analytic"], default="analytic",
+                    help="learned-step legs: Jacobian of the FP64 fallback")
+    ap.add_argument("--reference-fallback", type=int, default=1,
+                    help="also time both learned-step legs with the reference-eps FD fallback")
     ap.add_argument("--gate-elem-tol", type=float, default=None)
     ap.add_argument("--gate-elem-floor", type=float, default=None, help="mol/kg")
     ap.add_argument("--gate-dt-max", type=float, default=None, help="K")
@@ -277,13 +287,26 @@ def main():
         path = os.path.join(ROOT, "nrg_amd", "data", "surrogate_keromnes.npz")
         path = path if os.path.exists(path) else "none"
     if path != "none":
+        fb = (args.fallback_eps or None, args.fallback_jacobian)
         sur = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S,
-                            screen=args.screen)
+                            screen=args.screen, tuning=tuning, fallback=fb)
         if args.no_network:
             base = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S,
-                                 screen="only")
+                                 screen="only", tuning=tuning, fallback=fb)
             sur["no_network"] = base
             sur["network_gain"] = round(sur["value"] / base["value"], 4)
+        if args.reference_fallback and fb != (None, "fd"):
+            # the same legs with the fallback at the reference's eps and FD Jacobian
+            keep = ("ms_per_step", "speedup_vs_fp64", "fallback_ms", "fallback_nfe_per_step",
+                    "fallback_integrator", "audit_vs_fp64")
+            ref = {}
+            for name, scr in (("learned", args.screen), ("no_network", "only")):
+                if name == "no_network" and not args.no_network:
+                    continue
+                r = surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, value, S,
+                                  screen=scr, tuning=tuning, fallback=(None, "fd"))
+                ref[name] = {k: r[k] for k in keep}
+            sur["reference_fallback"] = ref
         sur["fallback_nfe_share_of_fp64"] = round(sur["fallback_nfe_per_step"] /
                                                   (nfe / args.steps), 4)
 
@@ -686,13 +709,15 @@ def table_leg(args, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S):
 
 
 def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_value, S,
-                  screen="rates"):
+                  screen="rates", tuning=None, fallback=(None, "fd")):
     """Time the learned split step (explicit-rate screen, network, gate, FP64
     fallback, closure) over the same number of steps from the same initial
     state, and audit its final state against the FP64 integrator's.  With
     screen='only' the network is not used: every cell the explicit-rate screen
     does not freeze goes to the FP64 integrator (the method without the
-    network)."""
+    network).  fallback = (eps, jacobian) of the FP64 fallback: (None, 'fd')
+    runs it on the FP64 leg's own handle (the reference's eps 1e-9), anything
+    else on a handle of its own with the FP64 leg's queue set-up."""
     import numpy as np
     import torch
 
@@ -718,7 +743,26 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_
     scr = {"none": capi.NRG_SUR_SCREEN_NONE, "rates": capi.NRG_SUR_SCREEN_RATES,
            "only": capi.NRG_SUR_SCREEN_ONLY}[screen]
     gate = sg.gate_struct(mech, screen=scr, **gkw)
-    chem = sg.SurrogateChemistry(solver, mlp, norm, gate)
+    fb_engine = None
+    fb_eps, fb_jac = fallback
+    if fb_eps is not None or fb_jac != "fd":
+        # a handle of its own for the fallback, with the FP64 leg's queue set-up
+        from nrg_amd.solver import ChemistryStep
+        b = tuning["best"] if tuning else {}
+        o = capi.default_opts(
+            form=b.get("form", capi.NRG_FORM_PERSISTENT if args.form == "persistent" else capi.NRG_FORM_PLAIN),
+            cell_order=b.get("cell_order", capi.NRG_ORDER_STIFFNESS if args.order == "stiffness"
+                             else capi.NRG_ORDER_NATURAL),
+            jac_batch_pct=b.get("jac_batch_pct", args.jac_pct), waves_per_cu=b.get("waves_per_cu", 0),
+            chunk_log2=(int(b["chunk_cells"]).bit_length() - 1) if b else args.chunk_log2,
+            errtest_gather_pct=b.get("errtest_gather_pct", args.gather_pct),
+            refill_min_idle=b.get("refill_min_idle", args.refill_idle), use_graph=args.graph)
+        if fb_eps is not None:
+            o.eps = fb_eps
+        if fb_jac == "analytic":
+            o.jacobian = capi.NRG_JAC_ANALYTIC
+        fb_engine = ChemistryStep(mech, o)
+    chem = sg.SurrogateChemistry(solver, mlp, norm, gate, engine=fb_engine)
 
     def step():
         st = chem.solve_chemical_kinetics(args.dt)
@@ -762,8 +806,13 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_
         "screen_ms": round(mean(lambda r: r.screen_ms), 3),
         "infer_ms": round(infer_ms, 3), "gate_ms": round(mean(lambda r: r.gate_ms), 3),
         "fallback_ms": round(mean(lambda r: r.fallback_ms), 3),
+        "fallback_integrator": {"eps": (fb_engine or solver.engine).opts.eps,
+                                "jacobian": "analytic" if (fb_engine or solver.engine).opts.jacobian
+                                == capi.NRG_JAC_ANALYTIC else "fd"},
         "audit_vs_fp64": audit_against(dm, T_ref, Y_ref, S, args.warmup + args.steps),
     }
+    if fb_engine is not None:
+        fb_engine.close()
     if mlp is not None:
         res["roofline_infer"] = {
             "bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_BF16_TFLOPS,

@@ -113,8 +113,12 @@ typedef struct nrg_surrogate_stats {
 /*
  * One chemistry step with the surrogate.  Arguments as nrg_chem_step_dev, plus
  * the trained network, its normalisation and the gate; `p` is required (the
- * network reads ln p).  Returns the FP64 fallback's fatal codes as
- * nrg_chem_step_dev does.
+ * network reads ln p; not with NRG_SUR_SCREEN_ONLY).  The FP64 fallback runs
+ * with the handle's integrator options: a handle created for the learned step
+ * with a looser eps and NRG_JAC_ANALYTIC (eps 1e-6 keeps a 200-step run within
+ * 1e-2 relative T and 1e-3 major-species mass fraction of the reference) gives
+ * a faster fallback; the screen and the gate do not depend on those options.
+ * Returns the FP64 fallback's fatal codes as nrg_chem_step_dev does.
  */
 int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surrogate_norm *norm,
                                 const nrg_surrogate_gate *gate, const nrg_grid *grid, double dt,

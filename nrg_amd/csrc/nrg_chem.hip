@@ -1829,6 +1829,8 @@ static int prepare_order(nrg_chem_t *h, long long n)
     h->hist_valid = false;
     HIP_TRY(hipMalloc(&h->order_list, (size_t)n * sizeof(int32_t)));
     HIP_TRY(hipMalloc(&h->hist, (size_t)n * sizeof(int32_t)));
+    // no step counts yet (never inside a capture: callers prepare before it)
+    HIP_TRY(hipMemsetAsync(h->hist, 0, (size_t)n * sizeof(int32_t), h->stream));
     h->order_cells = n;
   }
   if (!h->bins) HIP_TRY(hipMalloc(&h->bins, 2 * NBIN * sizeof(unsigned long long)));
@@ -2800,14 +2802,18 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     if (nfb > n) return set_error(NRG_CHEM_E_HIP, "fallback count %lld exceeds %lld cells", nfb, n);
     if (nfb > 0) {
       a.list = h->sur_list;
-      // the fallback cells are the stiffest of the field: with a stiffness
-      // history for this grid, hand them out most expensive first (and
-      // record their step counts for the next call)
-      if (h->order == NRG_ORDER_STIFFNESS && h->hist_valid && same_grid(h->hist_grid, a.g) &&
-          h->order_cells >= n) {
-        enqueue_order(h, a, true, h->sur_list, nfb);
+      // the fallback cells are the stiffest of the field: hand them out most
+      // expensive first, by the step counts this handle recorded for them
+      // (an FP64 step or an earlier fallback; temperature bins when it has
+      // none for this grid), and record their step counts for the next call
+      if (h->order == NRG_ORDER_STIFFNESS) {
+        if ((st = prepare_order(h, n))) return st;
+        const bool use_hist = h->hist_valid && same_grid(h->hist_grid, a.g);
+        enqueue_order(h, a, use_hist, h->sur_list, nfb);
         a.list = h->order_list;
         a.hist = h->hist;
+        h->hist_valid = true;
+        h->hist_grid = a.g;
       }
       if ((st = enqueue_integrator(h, a, nfb))) return st;
     }

@@ -6,7 +6,11 @@ Two copies of the same field are advanced side by side on the device, in
 NRG's layout:
 
 * the learned trajectory: the learned chemistry step (explicit-rate screen,
-  network, gate, FP64 fallback; include/nrg_surrogate.h) and the closure;
+  network, gate, FP64 fallback; include/nrg_surrogate.h) and the closure.
+  Its fallback integrates with `fallback_eps` and `fallback_jacobian`: by
+  default eps 1e-6 and the analytic Jacobian, a tolerance the run's error
+  bars leave room for (the shadow keeps the reference's eps 1e-9 and its
+  finite-difference Jacobian; None / 'fd' give the fallback those too);
 * the FP64 shadow: the FP64 chemistry step (nrg_chem_step_dev) and the
   same closure.
 
@@ -95,7 +99,8 @@ class _Trajectory:
 def run(shape=(512, 512, 256), steps=200, dt=1e-7, retrain_every=50, audit_frac=0.01,
         model=None, seed=2024, shift_max=(3, 2), fallback_pairs=1 << 15, shadow_pairs=1 << 15,
         retrain_epochs=2, retrain_lr=1e-4, retrain_batch=4096, gate_overrides=None,
-        screen="rates", autotune=True, log=print, device="cuda:0"):
+        screen="rates", autotune=True, log=print, device="cuda:0", fallback_eps=1e-6,
+        fallback_jacobian="analytic"):
     import torch
 
     from . import capi, inputs
@@ -111,7 +116,12 @@ def run(shape=(512, 512, 256), steps=200, dt=1e-7, retrain_every=50, audit_frac=
     ncells = nx * ny * nz
     st = inputs.hot_kernels_3d(mech, shape, seed=seed, device=dev)
     opts = dict(form=capi.NRG_FORM_PERSISTENT, cell_order=capi.NRG_ORDER_STIFFNESS, use_graph=1)
-    lrn = _Trajectory(shape, st, mech, dev, capi.default_opts(**opts))
+    lopts = capi.default_opts(**opts)
+    if fallback_eps is not None:
+        lopts.eps = fallback_eps
+    if fallback_jacobian == "analytic":
+        lopts.jacobian = capi.NRG_JAC_ANALYTIC
+    lrn = _Trajectory(shape, st, mech, dev, lopts)
     shd = _Trajectory(shape, st, mech, dev, capi.default_opts(**opts))
     del st
     g = lrn.dm.grid
@@ -307,6 +317,7 @@ def run(shape=(512, 512, 256), steps=200, dt=1e-7, retrain_every=50, audit_frac=
         "model": os.path.relpath(model, ROOT), "gate": gkw, "screen": screen,
         "shift_max_cells": list(shift_max), "audit_cells_per_step": n_aud,
         "retrain_every": retrain_every, "autotune": tuning,
+        "fallback_integrator": {"eps": lopts.eps, "jacobian": fallback_jacobian},
         "cell_steps_per_s_learned": ncells * steps / ms_l * 1e3,
         "cell_steps_per_s_fp64": ncells * steps / ms_f * 1e3,
         "speedup_vs_fp64": ms_f / ms_l,
@@ -350,13 +361,18 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--gate", action="append", default=[], metavar="K=V",
                     help="override a gate threshold of the model file, e.g. rate_tol=1e-4")
+    ap.add_argument("--fallback-eps", type=float, default=1e-6,
+                    help="integrator eps of the learned step's FP64 fallback (0: the "
+                         "reference's 1e-9)")
+    ap.add_argument("--fallback-jacobian", choices=["fd", "analytic"], default="analytic")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     over = dict((k, float(v)) for k, v in (g.split("=", 1) for g in a.gate))
     rep = run(tuple(a.shape), a.steps, a.dt, a.retrain_every, a.audit_frac, a.model, a.seed,
               shift_max=tuple(a.shift_max), retrain_epochs=a.retrain_epochs,
               retrain_lr=a.retrain_lr, gate_overrides=over, screen=a.screen,
-              log=lambda m: print(m, flush=True))
+              log=lambda m: print(m, flush=True), fallback_eps=a.fallback_eps or None,
+              fallback_jacobian=a.fallback_jacobian)
     line = json.dumps({k: v for k, v in rep.items() if k != "per_step"})
     print(line, flush=True)
     if a.out:

@@ -286,6 +286,34 @@ def test_rate_screen_without_network_is_the_freeze_test_plus_fp64(engine, oracle
     assert np.all(o["E_f_prod"][frozen] == 0)
 
 
+def test_fallback_takes_its_handles_integrator_options(engine, mech):
+    """The learned step on a handle created with a looser eps and the analytic
+    Jacobian: the screen freezes the same cells as on the reference handle (its
+    test does not involve the integrator), and every other live cell is that
+    handle's own FP64 step, bit for bit."""
+    from nrg_amd.solver import ChemistryStep
+    st, bc = _state(mech, seed=23)
+    dt = 1e-7
+    gate = _gate(mech, dY_freeze=1e-6, dT_freeze=0.05, screen=capi.NRG_SUR_SCREEN_ONLY)
+    loose = ChemistryStep(mech, capi.default_opts(eps=1e-6, jacobian=capi.NRG_JAC_ANALYTIC))
+    try:
+        a = _run(engine, None, None, gate, st, bc, dt)
+        b = _run(loose, None, None, gate, st, bc, dt)
+        f = _fp64(loose, st, bc, dt)
+        frozen = b["status"] == capi.NRG_CELL_SURROGATE
+        _report("loose_fallback", frozen=int(frozen.sum()), fallback=b["stats"].cells_fallback,
+                fallback_nfe=b["stats"].fallback.nfe_sum, reference_nfe=a["stats"].fallback.nfe_sum)
+        assert np.array_equal(frozen, a["status"] == capi.NRG_CELL_SURROGATE)
+        assert b["stats"].cells_fallback == a["stats"].cells_fallback > 0
+        for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+            assert np.array_equal(b[k][..., ~frozen], f[k][..., ~frozen]), k
+        # the looser integrator is the one in use: fewer right-hand sides, other answers
+        assert b["stats"].fallback.nfe_sum < a["stats"].fallback.nfe_sum
+        assert not np.array_equal(a["Y_prod"], b["Y_prod"])
+    finally:
+        loose.close()
+
+
 def test_rate_screen_then_network_equals_network_on_every_cell(engine, mech, zero_net):
     """gate.screen = RATES runs the network only on the cells the screen does not
     freeze.  With a network that answers zero, the gate freezes a cell exactly
