@@ -121,6 +121,16 @@ struct InlineMath {
   }
   static NRG_DI double log_(double x) { return log(x); }
   static NRG_DI double log10_(double x) { return log10(x); }
+  // x / T and x / R_u, as the reference divides
+  static NRG_DI double over_T(double x, double T) { return x / T; }
+  static NRG_DI double over_R(double x) { return x / R_GAS_J; }
+  // k_r = k_f / K_c with K_c = exp(lnKp) * x**d_nu (d_nu = 0: pow(x, 0) = 1 exactly)
+  static NRG_DI double reverse_(double kf, double lnKp, double x, double d_nu)
+  {
+    const double Kp = exp_(lnKp);
+    const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * pow_(x, d_nu);
+    return kf / Kc;
+  }
   // A T^b exp(mE_R / T), in the reference's order of operations
   static NRG_DI double arrh_(double A, double b, double mE_R, double T)
   {
@@ -145,9 +155,18 @@ struct EstimateMath {
   static NRG_DI double exp_(double x) { return exp(x); }
   static NRG_DI double log_(double x) { return log(x); }
   static NRG_DI double log10_(double x) { return log10(x); }
+  // divisions by T and R_u as products with one reciprocal per cell (1/T is
+  // the same expression at every call, formed once) and a constant
+  static NRG_DI double over_T(double x, double T) { return x * (1.0 / T); }
+  static NRG_DI double over_R(double x) { return x * (1.0 / R_GAS_J); }
+  // k_f / K_c as k_f exp(-lnKp) x**(-d_nu): no division
+  static NRG_DI double reverse_(double kf, double lnKp, double x, double d_nu)
+  {
+    return kf * exp(-lnKp) * ((d_nu == 0.0) ? 1.0 : pow_(x, -d_nu));
+  }
   static NRG_DI double arrh_(double A, double b, double mE_R, double T)
   {
-    return A * exp(b * log(T) + mE_R / T);
+    return A * exp(b * log(T) + mE_R * (1.0 / T));
   }
 };
 
@@ -282,6 +301,7 @@ constexpr int act_of(int s) { return (s >= 0 && s < S && kSp.act[s] >= 0) ? kSp.
 // ---------------------------------------------------------------------------
 NRG_DI double clip_t(double T) { return dmin(T, T_CEIL); }
 
+template <class M = InlineMath>
 NRG_DI double h_molar(const MechCoef &mc, double T, int s)
 {
   const double temp = clip_t(T);
@@ -292,7 +312,7 @@ NRG_DI double h_molar(const MechCoef &mc, double T, int s)
   hs = hs * temp + d[2];
   hs = hs * temp + d[3];
   hs = hs * temp + d[4];
-  hs = hs + mc.a[s][set][5] / temp;
+  hs = hs + M::over_T(mc.a[s][set][5], temp);
   return hs * temp;
 }
 
@@ -378,7 +398,7 @@ NRG_DI void reaction_k(const MechCoef &mc, double T, const double (&sv)[S], cons
     const double pr = (k0 * m) / kinf;
     const double fcent = (mc.one_m_alpha[r] * M::exp_(-(T / mc.troe[1][r])) +
                           mc.troe[0][r] * M::exp_(-(T / mc.troe[2][r]))) +
-                         M::exp_(-(mc.troe[3][r] / T));
+                         M::exp_(-M::over_T(mc.troe[3][r], T));
     const double lf = M::log10_(fcent);
     const double c = -0.4 - 0.67 * lf;
     const double n = 0.75 - 1.27 * lf;
@@ -394,10 +414,8 @@ NRG_DI void reaction_k(const MechCoef &mc, double T, const double (&sv)[S], cons
     kf = (kinf * (pr / (1.0 + pr))) * blend;
   }
   if (rt == 0 || rt == 1 || rt == 2 || rt == 3) {
-    const double Kp = M::exp_(d_s / R_GAS_J - (d_h / R_GAS_J) / T);
-    // K_c = K_p * ((P_atm/R_u)/T)**d_nu; d_nu = 0 gives pow(x, 0) = 1 exactly
-    const double Kc = (d_nu == 0.0) ? Kp * 1.0 : Kp * M::pow_(mc.patm_r / T, d_nu);
-    kr = kf / Kc;
+    // K_c = K_p * ((P_atm/R_u)/T)**d_nu, K_p = exp(dS/R - dH/(R T))
+    kr = M::reverse_(kf, M::over_R(d_s) - M::over_T(M::over_R(d_h), T), M::over_T(mc.patm_r, T), d_nu);
   }
   kf_out = kf;
   kr_out = kr;
@@ -500,7 +518,7 @@ NRG_DI void explicit_rates(const MechCoef &mc, double Tin, const double (&C)[S],
 #pragma unroll
   for (int c = 0; c < S; ++c) {
     sv[c] = s_molar<Mth>(mc, T, c);
-    hv[c] = h_molar(mc, T, c);
+    hv[c] = h_molar<Mth>(mc, T, c);
     absent |= (C[c] == 0.0) ? 1u << c : 0u;
   }
 #pragma unroll

@@ -1164,8 +1164,10 @@ __global__ void __launch_bounds__(256) rate_screen_kernel(SurArgs a)
 {
   __shared__ MechCoef mc;
   __shared__ double part[RCHECK_CELLS][SA];
+  __shared__ double rW[S];   // 1 / W_s: the estimate multiplies where the step divides
   __shared__ unsigned long long blk[SUR_SLOTS];
   if (threadIdx.x < SUR_SLOTS) blk[threadIdx.x] = 0ull;
+  if (threadIdx.x < S) rW[threadIdx.x] = 1.0 / a.mech->W[threadIdx.x];
   stage_mech(mc, a.mech);
   const int group = __builtin_amdgcn_readfirstlane(threadIdx.x / RCHECK_CELLS);
   const int lane = threadIdx.x & (WAVE - 1);
@@ -1181,7 +1183,7 @@ __global__ void __launch_bounds__(256) rate_screen_kernel(SurArgs a)
     rho = a.rho[off];
     T = a.T[off];
 #pragma unroll
-    for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) / mc.W[s];
+    for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) * rW[s];
     if (group == 0) explicit_rates<EstimateMath, 0, R / 2>(mc, T, C, Zd);
     else explicit_rates<EstimateMath, R / 2, R>(mc, T, C, Zd);
     if (group == 1) {
@@ -1194,15 +1196,16 @@ __global__ void __launch_bounds__(256) rate_screen_kernel(SurArgs a)
     bool frozen = false;
     if (live) {
       double dymax = 0.0, de = 0.0, cv = 0.0;
+      const double rrho = 1.0 / rho;
 #pragma unroll
       for (int i = 0; i < SA; ++i) {
         const int s = kSp.full[i];
-        const double est = (((Zd[i] + part[local][i]) * a.dt) * mc.W[s]) / rho;
+        const double est = (((Zd[i] + part[local][i]) * a.dt) * mc.W[s]) * rrho;
         dymax = fmax(dymax, fabs(est));
-        de += e_molar(mc, T, s) * (est / mc.W[s]);
+        de += e_molar(mc, T, s) * (est * rW[s]);
       }
 #pragma unroll
-      for (int s = 0; s < S; ++s) cv += (C[s] / rho) * (cp_molar(mc, T, s) - R_GAS_J);
+      for (int s = 0; s < S; ++s) cv += (C[s] * rrho) * (cp_molar(mc, T, s) - R_GAS_J);
       const double dT = -de / cv;
       frozen = a.dT_freeze > 0.0 && dymax <= a.dY_freeze && fabs(dT) <= a.dT_freeze;
       if (frozen) {
