@@ -2619,7 +2619,7 @@ int nrg_chem_autotune_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, const 
       c.gather = gather;
       if ((st = consider(c))) return st;
     }
-  for (int jac : {25, 50, 75, 100})
+  for (int jac : {10, 25, 50, 75, 100})
     if (jac != best.jac) {
       Cfg c = best;
       c.jac = jac;

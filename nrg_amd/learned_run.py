@@ -121,6 +121,7 @@ def run(shape=(512, 512, 256), steps=200, dt=1e-7, retrain_every=50, audit_frac=
         lopts.eps = fallback_eps
     if fallback_jacobian == "analytic":
         lopts.jacobian = capi.NRG_JAC_ANALYTIC
+        lopts.jac_batch_pct = 10   # no right-hand sides per Jacobian: batch it early
     lrn = _Trajectory(shape, st, mech, dev, lopts)
     shd = _Trajectory(shape, st, mech, dev, capi.default_opts(**opts))
     del st
