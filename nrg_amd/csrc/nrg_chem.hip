@@ -12,6 +12,13 @@
 // lives in its own namespace and hands its entry points to the C-ABI
 // (nrg_api.cpp) as a table (topo_api.h); nrg_chem_create picks the table whose
 // topology the mechanism has.
+//
+// NRG_PART splits a build into two objects: 1 = everything but the
+// integrator kernel's body, 2 = the integrator kernel alone (so that it can be
+// compiled with scheduling options of its own); 0 (default) = both in one.
+#ifndef NRG_PART
+#define NRG_PART 0
+#endif
 #ifndef NRG_TOPO_NS
 #define NRG_TOPO_NS nrg_topo_keromnes
 #define NRG_TOPO_OPS_FN nrg_topo_ops_keromnes
@@ -53,7 +60,7 @@ int nrg_chem_table_step_dev(nrg_chem_t *h, const nrg_grid *grid, double dt, cons
                             const int8_t *bc, double *const *Y_prod, double *const *Y_prod2,
                             double *E_f_prod, int32_t *cell_status, nrg_chem_stats *stats);
 
-namespace {
+namespace kern {
 
 // Field addressing of one call (Fortran column-major with ghost layers).
 struct GridIdx {
@@ -172,10 +179,12 @@ enum { ACC_INTEG = 0, ACC_NSTEP, ACC_NFE, ACC_NJE, ACC_NCORR, ACC_NATT, ACC_NSTE
 
 // Zero the work accumulators and the queue head (a kernel, so the launch
 // sequence can be captured in a graph without memset nodes).
+#if NRG_PART != 2
 __global__ void reset_kernel(unsigned long long *acc, int n)
 {
   if ((int)threadIdx.x < n) acc[threadIdx.x] = 0ull;
 }
+#endif
 
 typedef __attribute__((address_space(3))) void lds_void;
 typedef __attribute__((address_space(3))) double lds_double;
@@ -237,6 +246,7 @@ NRG_DI bool start_cell(const StepArgs &a, long long t, long long q, long long &o
   return true;
 }
 
+#if NRG_PART != 2
 // Cell records of the chunk's positions: the gate (:224-227), and for every
 // integrated cell its concentrations (:228-233), rate constants and [M]
 // (:235-252, 360-493), kept apart from the integrator so its ~80
@@ -291,6 +301,8 @@ __global__ void __launch_bounds__(256) rates_kernel(StepArgs a)
     }
   }
 }
+
+#endif
 
 // Output assembly (:318-340) once the cell's DDRIV3 has returned; returns the
 // cell's NRG_CELL_* status.
@@ -382,6 +394,13 @@ constexpr int REFILL_MIN_IDLE_DEFAULT = 63;   // measured: 48 -> 41.9, 56 -> 40.
 template <bool PERSISTENT, bool ANALYTIC>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(NRG_BDF_WAVES_PER_EU)))
 chem_bdf_kernel(StepArgs a)
+#if NRG_PART == 1
+;   // defined in the integrator part of this build
+extern template __global__ void chem_bdf_kernel<false, false>(StepArgs);
+extern template __global__ void chem_bdf_kernel<false, true>(StepArgs);
+extern template __global__ void chem_bdf_kernel<true, false>(StepArgs);
+extern template __global__ void chem_bdf_kernel<true, true>(StepArgs);
+#else
 {
   __shared__ BdfCoef mc;
   __shared__ __align__(16) double recl[2 * REC_LDS_PAIRS * WAVE];   // the lanes' cell records
@@ -466,6 +485,19 @@ chem_bdf_kernel(StepArgs a)
   }
   tl.flush(a.acc, lane);   // one atomic per wave and slot
 }
+#endif
+
+#if NRG_PART == 2
+// The integrator part ends here: its four kernels, compiled with their own
+// scheduling options (the Makefile), and nothing else.
+template __global__ void chem_bdf_kernel<false, false>(StepArgs);
+template __global__ void chem_bdf_kernel<false, true>(StepArgs);
+template __global__ void chem_bdf_kernel<true, false>(StepArgs);
+template __global__ void chem_bdf_kernel<true, true>(StepArgs);
+}  // namespace kern
+}  // namespace NRG_TOPO_NS
+#pragma GCC visibility pop
+#else
 
 // ---------------------------------------------------------------------------
 // Split-step closure and EOS (cabaret_solver.f90:3316-3318,3386-3416;
@@ -1422,7 +1454,8 @@ __global__ void __launch_bounds__(256) gate_kernel(SurArgs a)
     atomicAdd(&a.acc[threadIdx.x], blk[threadIdx.x]);
 }
 
-}  // namespace
+}  // namespace kern
+using namespace kern;
 
 // ===========================================================================
 // Scratch of nrg_chem_step_dev_noalloc: the work accumulators and queue heads
@@ -2890,3 +2923,4 @@ __attribute__((visibility("hidden"))) const nrg_topo_ops *NRG_TOPO_OPS_FN()
 #undef NRG_TOPO_OP_INIT
   return &table;
 }
+#endif  // NRG_PART != 2
