@@ -807,6 +807,7 @@ def surrogate_leg(args, path, mech, dm, solver, fields0, Y0, T_ref, Y_ref, fp64_
                               "reject_negative", "reject_energy", "reject_rate")},
         "gate": gkw,
         "screen_ms": round(mean(lambda r: r.screen_ms), 3),
+        "screen_fp64_frac": round(mean(lambda r: r.cells_screen_fp64) / ncells, 6),
         "infer_ms": round(infer_ms, 3), "gate_ms": round(mean(lambda r: r.gate_ms), 3),
         "fallback_ms": round(mean(lambda r: r.fallback_ms), 3),
         "fallback_integrator": {"eps": (fb_engine or solver.engine).opts.eps,

@@ -70,7 +70,12 @@ extern "C" {
  *   ONLY   the same screen, and every cell it does not freeze goes to the
  *          FP64 integrator: the method without the network (mlp and norm may
  *          be NULL, p is not read).
- * RATES and ONLY need dT_freeze > 0.
+ * RATES and ONLY need dT_freeze > 0.  The screen's decision is the FP64
+ * estimate's: a first pass evaluates the estimate with FP32 exponentials and a
+ * bound on its distance from the FP64 estimate, and decides each cell whose
+ * decision that bound settles; the FP64 estimate decides the others
+ * (stats.cells_screen_fp64).  gate.screen_fp64 = 1 sends every live cell to
+ * the FP64 estimate (same decisions, for comparison).
  */
 #define NRG_SUR_SCREEN_NONE 0
 #define NRG_SUR_SCREEN_RATES 1
@@ -93,7 +98,8 @@ typedef struct nrg_surrogate_gate {
   int32_t n_elements;                                /* rows of `elements` (<= 4)      */
   int32_t screen;                                    /* NRG_SUR_SCREEN_*, below        */
   double rate_tol;                                   /* mass fraction (0: off), below  */
-  int32_t reserved[4];
+  int32_t screen_fp64;                               /* 1: FP64 estimate for every cell */
+  int32_t reserved[3];
   double elements[NRG_SUR_MAX_ELEMENTS][32];         /* atoms of element e in species s */
 } nrg_surrogate_gate;
 
@@ -108,6 +114,7 @@ typedef struct nrg_surrogate_stats {
   int64_t reject_rate;        /* rejected by the rate-consistency test (rate_tol)      */
   int64_t cells_network;      /* cells the network ran on                              */
   double screen_ms;           /* the explicit-rate screen (SCREEN_RATES, SCREEN_ONLY)   */
+  int64_t cells_screen_fp64;  /* cells the screen decided with the FP64 estimate        */
 } nrg_surrogate_stats;
 
 /*

@@ -149,7 +149,7 @@ class NrgSurrogateGate(C.Structure):
                 ("energy_tol", C.c_double), ("dT_max", C.c_double),
                 ("dT_freeze", C.c_double), ("dY_freeze", C.c_double),
                 ("n_elements", C.c_int32), ("screen", C.c_int32), ("rate_tol", C.c_double),
-                ("reserved", C.c_int32 * 4),
+                ("screen_fp64", C.c_int32), ("reserved", C.c_int32 * 3),
                 ("elements", (C.c_double * 32) * NRG_SUR_MAX_ELEMENTS)]
 
 
@@ -162,7 +162,8 @@ class NrgSurrogateStats(C.Structure):
                 ("reject_confidence", C.c_int64), ("cells_frozen", C.c_int64),
                 ("infer_ms", C.c_double), ("gate_ms", C.c_double), ("fallback_ms", C.c_double),
                 ("fallback", NrgChemStats), ("reject_rate", C.c_int64),
-                ("cells_network", C.c_int64), ("screen_ms", C.c_double)]
+                ("cells_network", C.c_int64), ("screen_ms", C.c_double),
+                ("cells_screen_fp64", C.c_int64)]
 
 
 class NrgChemTuneTrial(C.Structure):

@@ -62,6 +62,8 @@ struct MechCoef {
   double el[MAXORD][MAXORD + 1]; // el[nq-1][j-1] = EL(j, nq)   (slatec/ddcst.f90, MINT = 2)
   double tq[MAXORD][3];          // tq[nq-1][k-1] = TQ(k, nq)
   double patm_r;                 // P_atm / R_u
+  double lnA[R], lnA_low[R];     // ln A, ln A_low (the explicit-rate screen's exponents)
+  double ln_patm_r;              // ln(P_atm / R_u)
 };
 
 // The part of the mechanism the integrator kernel reads (one LDS copy per block).

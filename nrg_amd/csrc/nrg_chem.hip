@@ -68,6 +68,7 @@ struct GridIdx {
   int nx, ny;                 // allocated extents in dims 1, 2
   int ilo0, ilo1, ilo2;       // inner lower bounds
   int ix, iy, iz;             // inner extents
+  int idx32;                  // 1: inner indices and allocation offsets below 2^31 (32-bit division)
   long long ninner;
   long long pad0, pad1;       // allocation offsets [pad0, pad1) holding every inner cell; pad0 % 64 == 0
 };
@@ -75,10 +76,19 @@ struct GridIdx {
 // Is allocation offset p an inner cell (not a ghost)?
 NRG_DI bool inner_at(const GridIdx &g, long long p)
 {
-  const long long r = p / g.nx;
-  const long long i = p - r * g.nx;
-  const long long j = r % g.ny;
-  const long long k = r / g.ny;
+  long long i, j, k;
+  if (g.idx32) {
+    const unsigned pu = (unsigned)p, nx = (unsigned)g.nx, ny = (unsigned)g.ny;
+    const unsigned r = pu / nx;
+    i = pu - r * nx;
+    j = r % ny;
+    k = r / ny;
+  } else {
+    const long long r = p / g.nx;
+    i = p - r * g.nx;
+    j = r % g.ny;
+    k = r / g.ny;
+  }
   return (unsigned long long)(i - (g.ilo0 - g.lo0)) < (unsigned long long)g.ix &&
          (unsigned long long)(j - (g.ilo1 - g.lo1)) < (unsigned long long)g.iy &&
          (unsigned long long)(k - (g.ilo2 - g.lo2)) < (unsigned long long)g.iz;
@@ -100,9 +110,18 @@ NRG_DI void at_inner_cell(const GridIdx &g, F &&f)
 
 NRG_DI long long cell_offset(const GridIdx &g, long long q)
 {
-  const long long i = q % g.ix + g.ilo0;
-  const long long j = (q / g.ix) % g.iy + g.ilo1;
-  const long long kk = q / ((long long)g.ix * g.iy) + g.ilo2;
+  long long i, j, kk;
+  if (g.idx32) {
+    const unsigned qu = (unsigned)q, ix = (unsigned)g.ix, iy = (unsigned)g.iy;
+    const unsigned r = qu / ix;
+    i = (long long)(qu - r * ix) + g.ilo0;
+    j = (long long)(r % iy) + g.ilo1;
+    kk = (long long)(r / iy) + g.ilo2;
+  } else {
+    i = q % g.ix + g.ilo0;
+    j = (q / g.ix) % g.iy + g.ilo1;
+    kk = q / ((long long)g.ix * g.iy) + g.ilo2;
+  }
   return (i - g.lo0) + (long long)g.nx * ((j - g.lo1) + (long long)g.ny * (kk - g.lo2));
 }
 
@@ -134,13 +153,24 @@ NRG_DI long long cell_of(const StepArgs &a, long long t)
   return a.list ? (long long)a.list[a.base + t] : a.base + t;
 }
 
-// Block-wide staging of the mechanism coefficients into LDS.
+// Block-wide staging of the mechanism coefficients into LDS: each thread
+// issues its loads (up to 4) before the first store, so a block waits for one
+// round trip to memory instead of one per element it copies.
 NRG_DI void stage_mech(MechCoef &dst, const MechCoef *src)
 {
   const double *s = reinterpret_cast<const double *>(src);
   double *d = reinterpret_cast<double *>(&dst);
   constexpr int n = sizeof(MechCoef) / sizeof(double);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = s[i];
+  constexpr int K = 4;
+  const int stride = blockDim.x;
+  for (int i0 = threadIdx.x; i0 < n; i0 += K * stride) {
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = (i0 + k * stride < n) ? s[i0 + k * stride] : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (i0 + k * stride < n) d[i0 + k * stride] = v[k];
+  }
   __syncthreads();
 }
 
@@ -996,7 +1026,7 @@ __global__ void __launch_bounds__(BIN_THREADS) bin_scatter_kernel(BinArgs a)
 constexpr int NIN = S + 3, NOUT = S + 1;
 
 enum { SUR_FALLBACK = 0, SUR_ACCEPT, SUR_GATED, SUR_R_DOMAIN, SUR_R_ELEM, SUR_R_NEG, SUR_R_ENERGY,
-       SUR_R_CONF, SUR_R_RATE, SUR_FROZEN, SUR_CAND, SUR_SLOTS = 11 };
+       SUR_R_CONF, SUR_R_RATE, SUR_FROZEN, SUR_CAND, SUR_RECHECK, SUR_SLOTS = 12 };
 constexpr int SUR_NREASONS = SUR_R_RATE - SUR_R_DOMAIN + 1;
 
 struct SurArgs {
@@ -1015,8 +1045,11 @@ struct SurArgs {
   double E[NRG_SUR_MAX_ELEMENTS][S];
   const int32_t *cand;        // NULL: position p is inner cell p; else inner cell cand[p]
   long long npos;             // positions the network, rate check and gate visit
-  int32_t *screen_out;        // rate_screen_kernel: where the cells it does not freeze go
+  int32_t *screen_out;        // the screen: where the cells it does not freeze go
   int screen_slot;            // ... and the counter that numbers them (SUR_CAND, SUR_FALLBACK)
+  int screen_fp64;            // 1: the FP64 estimate decides every live cell
+  int32_t *recheck;           // cells screen_kernel leaves to rate_screen_kernel ...
+  const unsigned long long *recheck_n;   // ... and their number (acc[SUR_RECHECK])
   bf16 *X;                    // npos x MLP_IN_PAD
   const float *Ypred;         // npos x NOUT
   double *Yp[S], *Yp2[S];
@@ -1192,8 +1225,45 @@ __global__ void __launch_bounds__(256) rate_check_kernel(SurArgs a)
 // live cell is appended to screen_out: the network's candidates, or, for the
 // method without the network, the FP64 fallback list.  Gated cells get the
 // FP64 step's outputs for a skipped cell (chemical_kinetics_solver.f90:341-346).
+//
+// Two kernels share the work, and the decision is always the one the FP64
+// estimate gives.  screen_kernel visits every inner cell and evaluates the
+// estimate with FP64 exponents and FP32 exponentials (the transcendentals
+// that make up most of the FP64 estimate's cost), carrying a bound on its
+// distance from the FP64 estimate.  Where the bound settles the decision it
+// decides; the other live cells go to a list, which rate_screen_kernel then
+// decides with the FP64 estimate itself.
+
+NRG_DI void freeze_outputs(const SurArgs &a, long long off)
+{
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    a.Yp[s][off] = 0.0;
+    a.Yp2[s][off] = 0.0;
+  }
+  a.Efp[off] = 0.0;
+  if (a.status) a.status[off] = NRG_CELL_SURROGATE;
+}
+
+// Appends the wave's lanes with `take` to out[] at the counter *ctr.
+NRG_DI void wave_append(bool take, int lane, long long q, int32_t *out, unsigned long long *ctr)
+{
+  const unsigned long long mask = __ballot(take ? 1 : 0);
+  if (!mask) return;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader, WAVE);
+  if (take) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)q;
+}
+
+// The FP64 decision on the cells screen_kernel lists (recheck[0, *recheck_n)).
+// Waves 0-1 and 2-3 of a block take the two halves of the reactions for the
+// same 128 cells, as in rate_check_kernel; blocks stride over the list.
 __global__ void __launch_bounds__(256) rate_screen_kernel(SurArgs a)
 {
+  const long long npos = (long long)*a.recheck_n;
+  if ((long long)blockIdx.x * RCHECK_CELLS >= npos) return;   // the whole block
   __shared__ MechCoef mc;
   __shared__ double part[RCHECK_CELLS][SA];
   __shared__ double rW[S];   // 1 / W_s: the estimate multiplies where the step divides
@@ -1204,77 +1274,371 @@ __global__ void __launch_bounds__(256) rate_screen_kernel(SurArgs a)
   const int group = __builtin_amdgcn_readfirstlane(threadIdx.x / RCHECK_CELLS);
   const int lane = threadIdx.x & (WAVE - 1);
   const int local = threadIdx.x % RCHECK_CELLS;
-  const long long q = (long long)blockIdx.x * RCHECK_CELLS + local;
-  const bool valid = q < a.g.ninner;
-  const long long off = valid ? cell_offset(a.g, q) : 0;
-  const bool gated = valid && gated_cell(a, off);
-  const bool live = valid && !gated;
-  double C[S], Zd[SA];
-  double rho = 1.0, T = 0.0;
-  if (live) {
-    rho = a.rho[off];
-    T = a.T[off];
+  for (long long p0 = (long long)blockIdx.x * RCHECK_CELLS; p0 < npos;
+       p0 += (long long)gridDim.x * RCHECK_CELLS) {
+    // the coefficients in LDS are read where they are used, not held in
+    // registers across the loop
+    asm volatile("" ::: "memory");
+    const long long pos = p0 + local;
+    const bool live = pos < npos;   // listed cells are live
+    const long long q = live ? (long long)a.recheck[pos] : 0;
+    const long long off = live ? cell_offset(a.g, q) : 0;
+    double C[S], Zd[SA];
+    double rho = 1.0, T = 0.0;
+    if (live) {
+      rho = a.rho[off];
+      T = a.T[off];
 #pragma unroll
-    for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) * rW[s];
-    if (group == 0) explicit_rates<EstimateMath, 0, R / 2>(mc, T, C, Zd);
-    else explicit_rates<EstimateMath, R / 2, R>(mc, T, C, Zd);
-    if (group == 1) {
+      for (int s = 0; s < S; ++s) C[s] = (a.Y[s][off] * rho) * rW[s];
+      if (group == 0) explicit_rates<EstimateMath, 0, R / 2>(mc, T, C, Zd);
+      else explicit_rates<EstimateMath, R / 2, R>(mc, T, C, Zd);
+      if (group == 1) {
 #pragma unroll
-      for (int i = 0; i < SA; ++i) part[local][i] = Zd[i];
+        for (int i = 0; i < SA; ++i) part[local][i] = Zd[i];
+      }
+    }
+    __syncthreads();
+    if (group == 0) {
+      bool frozen = false;
+      if (live) {
+        double dymax = 0.0, de = 0.0, cv = 0.0;
+        const double rrho = 1.0 / rho;
+#pragma unroll
+        for (int i = 0; i < SA; ++i) {
+          const int s = kSp.full[i];
+          const double est = (((Zd[i] + part[local][i]) * a.dt) * mc.W[s]) * rrho;
+          dymax = fmax(dymax, fabs(est));
+          de += e_molar(mc, T, s) * (est * rW[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) cv += (C[s] * rrho) * (cp_molar(mc, T, s) - R_GAS_J);
+        const double dT = -de / cv;
+        frozen = a.dT_freeze > 0.0 && dymax <= a.dY_freeze && fabs(dT) <= a.dT_freeze;
+        if (frozen) freeze_outputs(a, off);
+      }
+      wave_append(live && !frozen, lane, q, a.screen_out, &a.acc[a.screen_slot]);
+      const unsigned long long c_frz = __popcll(__ballot(live && frozen ? 1 : 0));
+      if (lane == 0 && c_frz) {
+        atomicAdd(&blk[SUR_ACCEPT], c_frz);
+        atomicAdd(&blk[SUR_FROZEN], c_frz);
+      }
+    }
+    __syncthreads();   // part[] is rewritten by the next slice
+  }
+  if (threadIdx.x < SUR_SLOTS && (int)threadIdx.x != a.screen_slot && blk[threadIdx.x])
+    atomicAdd(&a.acc[threadIdx.x], blk[threadIdx.x]);
+}
+
+// ln x for x > 0 (normal or subnormal): frexp's exponent plus an FP32 log2 of
+// the mantissa m in [0.5, 1).  Rounding m to FP32 moves log2 m by at most
+// 2^-24 / ln 2 and log2f is within 2 ulp (2^-23 on [-1, 0]), so
+// |ln_m32(x) - ln x| < 2^-22, bounded below by SCR_LN_ERR.
+constexpr double SCR_LN_ERR = 0x1p-20;
+constexpr double SCR_LN2 = 0.69314718055994530942;
+constexpr double SCR_LOG2E = 1.4426950408889634074;
+constexpr double SCR_LN10 = 2.30258509299404568402;
+
+NRG_DI double ln_m32(double x)
+{
+  int e;
+  const double m = frexp(x, &e);
+  return ((double)e + (double)log2f((float)m)) * SCR_LN2;
+}
+
+// v = e^x as FP32 exp2 of an FP64 exponent x that is within ex of the exact
+// one, and err, a bound on |v - e^x_exact|: rounding x log2(e) to FP32 moves
+// the value by at most |x| 2^-24 relative and exp2f is within 2 ulp (2^-22),
+// here taken as 2^-23 |x| and 2^-21; e^ex - 1 <= 1.01 ex for ex <= 0.01; and
+// FP32 results below its normal range add at most 2^-125 (exponents below
+// -100 give the value 0 with that bound).  Returns false where the value
+// overflows FP32, is NaN, or the bound exceeds 1 %.
+NRG_DI bool exp_term(double x, double ex, double &v, double &err)
+{
+  if (x == -INFINITY) {   // a zero rate constant or third-body concentration
+    v = 0.0;
+    err = 0.0;
+    return true;
+  }
+  if (x < -100.0) {       // below FP32's range: e^(x + ex) < e^-99 < 2^-125 for ex <= 1
+    v = 0.0;
+    err = 0x1p-125;
+    return ex <= 1.0;
+  }
+  const double rel = ex + fabs(x) * 0x1p-23 + 0x1p-21;
+  const float y = exp2f((float)(x * SCR_LOG2E));
+  v = (double)y;
+  err = rel * 1.01 * v + 0x1p-125;
+  return y <= 3.0e38f && rel <= 0.01;
+}
+
+// Stoichiometry of species s in reaction r: right minus left count, and the
+// number of times it appears (the bound adds a term's error once per
+// appearance, as the right-hand side adds the term).
+constexpr int nu_net(int r, int s)
+{
+  int n = 0;
+  for (int c = 0; c < 3; ++c) {
+    if (c < topo::nr[r] && topo::right[r][c] == s) ++n;
+    if (c < topo::nl[r] && topo::left[r][c] == s) --n;
+  }
+  return n;
+}
+constexpr int nu_occ(int r, int s)
+{
+  int n = 0;
+  for (int c = 0; c < 3; ++c) {
+    if (c < topo::nr[r] && topo::right[r][c] == s) ++n;
+    if (c < topo::nl[r] && topo::left[r][c] == s) ++n;
+  }
+  return n;
+}
+
+// Screen estimate of one cell: FP64 exponents (ln k_f, ln K_p, ln [M], ln C
+// from ln_m32) and FP32 exponentials.  acc[i] = sum over the reactions of
+// nu (F - B) for active species i, bnd[i] a bound on its distance from the
+// FP64 estimate's sum.  The reactions' arithmetic follows reaction_k (rate
+// constants, chemical_kinetics_solver.f90:394-485) and rhs_reaction (mass
+// action, :507-536) in logarithms:
+//   ln k_f = ln A + b ln T - E/(R T)
+//   fall-off: ln k_f = ln k_inf - ln(1 + 1/P_r) [+ ln F for Troe, with
+//             ln F = ln F_cent / (1 + x^2)], ln P_r = ln k_0 + ln [M] - ln k_inf
+//   ln k_r = ln k_f - ln K_p + d_nu ln(R_u T / P_atm)
+// Returns false (the FP64 path decides the cell) where a term leaves FP32's
+// range, a bound exceeds 1 % or a quantity the logarithms need is not
+// positive (a negative concentration or [M], a Troe denominator near zero).
+NRG_DI bool screen_estimate(const MechCoef &mc, double Tin, const double (&C)[S], double (&acc)[SA],
+                            double (&bnd)[SA], double (&H)[S])
+{
+  const double T = dmin(Tin, 10000.0);
+  const double lnT = log(T), rT = 1.0 / T;
+  const double ln_x = mc.ln_patm_r - lnT;   // ln(P_atm / (R_u T))
+  bool ok = true;
+  double lnC[S], G[S];                    // G_s = s_s/R - h_s/(R T): ln K_p = sum nu G
+  unsigned absent = 0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    // one species at a time: its coefficients are read after the previous
+    // species' values are formed (not all species' coefficients at once)
+    asm volatile("" ::: "memory");
+    ok = ok && C[s] >= 0.0;
+    absent |= (C[s] == 0.0) ? 1u << s : 0u;
+    lnC[s] = C[s] > 0.0 ? ln_m32(C[s]) : 0.0;
+    H[s] = h_molar<EstimateMath>(mc, T, s);
+    G[s] = s_molar<EstimateMath>(mc, T, s) * (1.0 / R_GAS_J) - (H[s] * (1.0 / R_GAS_J)) * rT;
+    asm volatile("" : "+v"(G[s]), "+v"(lnC[s]), "+v"(H[s]));
+  }
+#pragma unroll
+  for (int i = 0; i < SA; ++i) {
+    acc[i] = 0.0;
+    bnd[i] = 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool fwd = (absent & side_mask(r, 0)) == 0;
+    const bool rev = has_reverse(r) && (absent & side_mask(r, 1)) == 0;
+    if (!__any((fwd || rev) ? 1 : 0)) continue;
+    const int rt = topo::rtype[r];
+    double m = 0.0, lm = 0.0;
+    if (uses_m(r)) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) m = m + C[s] * mc.eff[r][s];
+      lm = m > 0.0 ? ln_m32(m) : -INFINITY;
+    }
+    // ln k_f and a bound on its error
+    // (rok: this reaction's checks, which matter only where the cell has a
+    // term of it)
+    double lkf = mc.lnA[r] + mc.beta[r] * lnT + mc.mE_R[r] * rT, ekf = 0.0;
+    bool rok = true;
+    if (rt == 2 || rt == 3 || rt == 7) {
+      const double lk0 = mc.lnA_low[r] + mc.beta_low[r] * lnT + mc.mElow_R[r] * rT;
+      // P_r = 0 ([M] = 0: every efficiency of the cell's species zero, or
+      // k_0 = 0): k_f = k_inf * 0 * F = 0 exactly in the FP64 estimate too
+      const bool zero_pr = m == 0.0 || lk0 == -INFINITY;
+      const double lpr = zero_pr ? 0.0 : lk0 + lm - lkf;
+      rok = m >= 0.0 && lpr > -1e4 && lpr < 1e4;
+      // ln(P_r / (1 + P_r)) = -softplus(-ln P_r) in FP32: the argument's
+      // rounding, expf and log1pf (2 ulp each) and the sum stay within
+      // (|ln P_r| + 1) 2^-23 + 2^-21; ln [M] adds SCR_LN_ERR
+      const float z = (float)(-lpr);
+      const float sp = z > 0.0f ? z + log1pf(expf(-z)) : log1pf(expf(z));
+      lkf = lkf - (double)sp;
+      ekf = SCR_LN_ERR + (fabs(lpr) + 1.0) * 0x1p-23 + 0x1p-21;
+      if (rt == 3) {
+        double e1, e2, e3, r1, r2, r3;
+        rok = exp_term(-(T / mc.troe[1][r]), 0.0, e1, r1) && rok;
+        rok = exp_term(-(T / mc.troe[2][r]), 0.0, e2, r2) && rok;
+        rok = exp_term(-(mc.troe[3][r] * rT), 0.0, e3, r3) && rok;
+        const double fcent = (mc.one_m_alpha[r] * e1 + mc.troe[0][r] * e2) + e3;
+        const double efc = fabs(mc.one_m_alpha[r]) * r1 + fabs(mc.troe[0][r]) * r2 + r3;
+        // (with P_r = 0 the FP64 estimate multiplies 0 by F_cent^(-1/0.14),
+        // finite for F_cent above 1e-40)
+        rok = rok && fcent > 0.0 && efc <= 0.001 * fcent && (!zero_pr || fcent > 1e-40);
+        const double lnfc = ln_m32(fcent > 0.0 ? fcent : 1.0);
+        const double elnfc = SCR_LN_ERR + 1.01 * efc / (fcent > 0.0 ? fcent : 1.0);
+        const double lf = lnfc * (1.0 / SCR_LN10);
+        const double cc = -0.4 - 0.67 * lf, nn = 0.75 - 1.27 * lf;
+        const double nume = lpr * (1.0 / SCR_LN10) + cc;
+        const double den = nn - 0.14 * nume;
+        const double xx = nume / den;
+        // perturbations of the numerator and denominator of x from ln [M]
+        // and ln F_cent, and the resulting bound on |x - x_FP64|; the
+        // derivative of 1 / (1 + x^2) is at most 0.65 in magnitude
+        const double elf = elnfc * (1.0 / SCR_LN10);
+        const double dnum = SCR_LN_ERR * (1.0 / SCR_LN10) + 0.67 * elf;
+        const double dden = 1.27 * elf + 0.14 * dnum;
+        rok = rok && (zero_pr || (fabs(den) > 0.0625 && fabs(den) > 2.0 * dden));
+        const double adn = fabs(den) > 2.0 * dden ? fabs(den) : 1.0;
+        const double exx = (dnum * adn + fabs(nume) * dden) / (adn * (adn - dden));
+        lkf = lkf + lnfc / (1.0 + xx * xx);
+        ekf = ekf + fabs(lnfc) * 0.65 * exx + elnfc;
+      }
+      if (zero_pr) lkf = -INFINITY;
+    }
+    // forward and reverse terms of the mass action
+    double xf = lkf, ef = ekf;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int sp = topo::left[r][c];
+      if (c < topo::nl[r]) {
+        xf = xf + ((sp == S) ? lm : lnC[sp < S ? sp : 0]);
+        ef = ef + SCR_LN_ERR;
+      }
+    }
+    ok = ok && (rok || !(fwd || rev));
+    double F = 0.0, eF = 0.0;
+    if (fwd) ok = exp_term(xf, ef, F, eF) && ok;
+    double B = 0.0, eB = 0.0;
+    if (has_reverse(r)) {
+      double lnKp = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int sp = topo::right[r][c];
+        if (c < topo::nr[r] && sp != S) lnKp = lnKp + G[sp < S ? sp : 0];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int sp = topo::left[r][c];
+        if (c < topo::nl[r] && sp != S) lnKp = lnKp - G[sp < S ? sp : 0];
+      }
+      const double d_nu = (double)(topo::nr[r] - topo::nl[r]);
+      double xb = (lkf - lnKp) - d_nu * ln_x, eb = ekf;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int sp = topo::right[r][c];
+        if (c < topo::nr[r]) {
+          xb = xb + ((sp == S) ? lm : lnC[sp < S ? sp : 0]);
+          eb = eb + SCR_LN_ERR;
+        }
+      }
+      if (rev) ok = exp_term(xb, eb, B, eB) && ok;
+    }
+#pragma unroll
+    for (int i = 0; i < SA; ++i) {
+      const int s = kSp.full[i];
+      if (nu_net(r, s) != 0) acc[i] = acc[i] + (double)nu_net(r, s) * (F - B);
+      if (nu_occ(r, s) != 0) bnd[i] = bnd[i] + (double)nu_occ(r, s) * (eF + eB);
     }
   }
-  __syncthreads();
-  if (group == 0) {
-    bool frozen = false;
-    if (live) {
-      double dymax = 0.0, de = 0.0, cv = 0.0;
-      const double rrho = 1.0 / rho;
+  return ok;
+}
+
+enum { SCR_FROZEN = 0, SCR_PASS = 1, SCR_RECHECK = 2 };   // screen_kernel's outcome for a live cell
+
+// The screen over every inner cell (a block per 256 cells, one cell per
+// thread; a loop over slices would hold the expansions' FP64 constants in
+// registers across it and halve the resident waves).  Outcomes: gated, frozen, passed on (screen_out), or
+// listed for rate_screen_kernel (recheck).  With screen_fp64 every live cell
+// is listed, so the FP64 estimate decides every cell.
+__global__ void __launch_bounds__(256) screen_kernel(SurArgs a)
+{
+  __shared__ MechCoef mc;
+  __shared__ double rW[S];
+  __shared__ unsigned long long blk[SUR_SLOTS];
+  if (threadIdx.x < SUR_SLOTS) blk[threadIdx.x] = 0ull;
+  if (threadIdx.x < S) rW[threadIdx.x] = 1.0 / a.mech->W[threadIdx.x];
+  stage_mech(mc, a.mech);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const long long n = a.g.ninner;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = q < n;
+  const long long off = valid ? cell_offset(a.g, q) : 0;
+  // every input in one round trip to memory (the loads do not wait for the
+  // gate test, as in gated_cell)
+  double T = 0.0, rho = 1.0, y[S];
+  int8_t bcv = 0;
+  if (valid) {
+    T = a.T[off];
+    bcv = a.bc ? a.bc[off] : (int8_t)0;
+    rho = a.rho[off];
 #pragma unroll
-      for (int i = 0; i < SA; ++i) {
-        const int s = kSp.full[i];
-        const double est = (((Zd[i] + part[local][i]) * a.dt) * mc.W[s]) * rrho;
-        dymax = fmax(dymax, fabs(est));
-        de += e_molar(mc, T, s) * (est * rW[s]);
-      }
+    for (int s = 0; s < S; ++s) y[s] = a.Y[s][off];
+  }
+  const bool gated = valid && !(T >= a.t_gate && bcv == 0);
+  const bool live = valid && !gated;
+  int outcome = SCR_PASS;
+  if (live) {
+    double C[S], acc[SA], bnd[SA], H[S];
 #pragma unroll
-      for (int s = 0; s < S; ++s) cv += (C[s] * rrho) * (cp_molar(mc, T, s) - R_GAS_J);
-      const double dT = -de / cv;
-      frozen = a.dT_freeze > 0.0 && dymax <= a.dY_freeze && fabs(dT) <= a.dT_freeze;
-      if (frozen) {
+    for (int s = 0; s < S; ++s) C[s] = (y[s] * rho) * rW[s];
+    const bool ok = screen_estimate(mc, T, C, acc, bnd, H);
+    // the decision with the FP64 estimate's expressions, each side of each
+    // threshold taken only where the bound puts the FP64 estimate there too
+    const double rrho = 1.0 / rho;
+    bool in = true, out = false;
+    double de = 0.0, bde = 0.0, dabs = 0.0, cv = 0.0;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          a.Yp[s][off] = 0.0;
-          a.Yp2[s][off] = 0.0;
-        }
-        a.Efp[off] = 0.0;
-        if (a.status) a.status[off] = NRG_CELL_SURROGATE;
-      }
+    for (int i = 0; i < SA; ++i) {
+      const int s = kSp.full[i];
+      const double est = ((acc[i] * a.dt) * mc.W[s]) * rrho;
+      asm volatile("" ::: "memory");   // as in screen_estimate
+      const double b = (((bnd[i] * a.dt) * mc.W[s]) * rrho) * (1.0 + 0x1p-40);
+      in = in && fabs(est) + b <= a.dY_freeze;
+      out = out || fabs(est) - b > a.dY_freeze;
+      // e_s as e_molar forms it, from h_s with a5 * (1/T) where h_molar
+      // divides: within 2^-50 (|a5| + |h_s|) of the FP64 estimate's e_s
+      const double e = H[s] - R_GAS_J * T;
+      de += e * (est * rW[s]);
+      bde += fabs(e) * (b * rW[s]);
+      const double ea = fabs(e) + 0x1p-10 * (fabs(mc.a[s][0][5]) + fabs(mc.a[s][1][5]) + fabs(H[s]));
+      dabs += ea * ((fabs(est) + b) * rW[s]);
+      asm volatile("" : "+v"(de), "+v"(bde), "+v"(dabs));
     }
-    if (gated) {
 #pragma unroll
-      for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
-      a.Efp[off] = 0.0;
-      if (a.status) a.status[off] = NRG_CELL_GATED;
+    for (int s = 0; s < S; ++s) {
+      asm volatile("" ::: "memory");
+      cv += (C[s] * rrho) * (cp_molar(mc, T, s) - R_GAS_J);
+      asm volatile("" : "+v"(cv));
     }
-    const bool pass = live && !frozen;
-    const unsigned long long mask = __ballot(pass ? 1 : 0);
-    if (mask) {
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(&a.acc[a.screen_slot], (unsigned long long)__popcll(mask));
-      base = __shfl(base, leader, WAVE);
-      if (pass) a.screen_out[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)q;
-    }
-    const unsigned long long c_frz = __popcll(__ballot(live && frozen ? 1 : 0));
-    const unsigned long long c_gat = __popcll(__ballot(gated ? 1 : 0));
-    if (lane == 0) {
+    // bT bounds |dT - dT_FP64|: the bounds on the estimates, the two e_s
+    // and the FP64 roundings of both sums (dabs)
+    const double dT = -de / cv;
+    const double bT = (bde + dabs * 0x1p-40) / cv + fabs(dT) * 0x1p-40;
+    const bool sane = ok && cv > 0.0 && bT < INFINITY && fabs(dT) < INFINITY;
+    in = in && fabs(dT) + bT <= a.dT_freeze;
+    out = out || fabs(dT) - bT > a.dT_freeze;
+    outcome = (!sane || a.screen_fp64) ? SCR_RECHECK : out ? SCR_PASS : in ? SCR_FROZEN : SCR_RECHECK;
+    if (outcome == SCR_FROZEN) freeze_outputs(a, off);
+  }
+  if (gated) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) a.Yp[s][off] = 0.0;
+    a.Efp[off] = 0.0;
+    if (a.status) a.status[off] = NRG_CELL_GATED;
+  }
+  wave_append(live && outcome == SCR_PASS, lane, q, a.screen_out, &a.acc[a.screen_slot]);
+  wave_append(live && outcome == SCR_RECHECK, lane, q, a.recheck, &a.acc[SUR_RECHECK]);
+  const unsigned long long c_frz = __popcll(__ballot(live && outcome == SCR_FROZEN ? 1 : 0));
+  const unsigned long long c_gat = __popcll(__ballot(gated ? 1 : 0));
+  if (lane == 0) {
+    if (c_frz) {
       atomicAdd(&blk[SUR_ACCEPT], c_frz);
       atomicAdd(&blk[SUR_FROZEN], c_frz);
-      atomicAdd(&blk[SUR_GATED], c_gat);
     }
+    if (c_gat) atomicAdd(&blk[SUR_GATED], c_gat);
   }
   __syncthreads();
-  if (threadIdx.x < SUR_SLOTS && (int)threadIdx.x != a.screen_slot && blk[threadIdx.x])
+  if (threadIdx.x < SUR_SLOTS && (int)threadIdx.x != a.screen_slot && (int)threadIdx.x != SUR_RECHECK &&
+      blk[threadIdx.x])
     atomicAdd(&a.acc[threadIdx.x], blk[threadIdx.x]);
 }
 
@@ -1513,7 +1877,9 @@ struct nrg_chem_handle {
   float *sur_Y = nullptr;
   int32_t *sur_list = nullptr;
   int32_t *sur_cand = nullptr;             // cells the explicit-rate screen hands to the network
+  int32_t *sur_recheck = nullptr;          // cells its FP32 pass leaves to the FP64 estimate
   int8_t *sur_rate_bad = nullptr;
+  long long recheck_blocks = 0;            // resident blocks of rate_screen_kernel
   unsigned long long *sur_acc = nullptr;
   long long sur_cells = 0;
   // stiffness-ordered queue
@@ -1624,6 +1990,11 @@ int build_mech(const nrg_chem_mech *m, MechCoef &mc)
   }
   gear_tables(mc.el, mc.tq);
   mc.patm_r = 101325.0 / R_GAS_J;
+  for (int r = 0; r < R; ++r) {
+    mc.lnA[r] = std::log(mc.A[r]);
+    mc.lnA_low[r] = std::log(mc.A_low[r]);
+  }
+  mc.ln_patm_r = std::log(mc.patm_r);
   return NRG_CHEM_OK;
 }
 
@@ -1676,6 +2047,7 @@ int make_grid(const nrg_grid *grid, GridIdx &g)
     g.pad0 = off(grid->inner_lo[0], grid->inner_lo[1], grid->inner_lo[2]) & ~63ll;
     g.pad1 = off(grid->inner_hi[0], grid->inner_hi[1], grid->inner_hi[2]) + 1;
   }
+  g.idx32 = g.ninner < (1ll << 31) && g.pad1 < (1ll << 31);
   return NRG_CHEM_OK;
 }
 
@@ -1837,7 +2209,7 @@ void nrg_chem_destroy(nrg_chem_t *h)
   for (hipEvent_t e : {h->ev0, h->ev1, h->ev2, h->ev3})
     if (e) (void)hipEventDestroy(e);
   for (void *p : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_rate_bad,
-                  (void *)h->sur_cand, (void *)h->sur_acc,
+                  (void *)h->sur_cand, (void *)h->sur_recheck, (void *)h->sur_acc,
                   (void *)h->order_list, (void *)h->hist, (void *)h->bins, (void *)h->table})
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -2763,16 +3135,17 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
   const long long n = a.g.ninner;
   if (n > h->sur_cells) {
     for (void *q : {(void *)h->sur_X, (void *)h->sur_Y, (void *)h->sur_list, (void *)h->sur_rate_bad,
-                    (void *)h->sur_cand})
+                    (void *)h->sur_cand, (void *)h->sur_recheck})
       if (q) HIP_TRY(hipFree(q));
     h->sur_X = nullptr; h->sur_Y = nullptr; h->sur_list = nullptr; h->sur_rate_bad = nullptr;
-    h->sur_cand = nullptr;
+    h->sur_cand = nullptr; h->sur_recheck = nullptr;
     h->sur_cells = 0;
     HIP_TRY(hipMalloc(&h->sur_X, (size_t)n * MLP_IN_PAD * sizeof(bf16)));
     HIP_TRY(hipMalloc(&h->sur_Y, (size_t)n * NOUT * sizeof(float)));
     HIP_TRY(hipMalloc(&h->sur_list, (size_t)n * sizeof(int32_t)));
     HIP_TRY(hipMalloc(&h->sur_rate_bad, (size_t)n * sizeof(int8_t)));
     HIP_TRY(hipMalloc(&h->sur_cand, (size_t)n * sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&h->sur_recheck, (size_t)n * sizeof(int32_t)));
     h->sur_cells = n;
   }
   if (!h->sur_acc) HIP_TRY(hipMalloc(&h->sur_acc, SUR_SLOTS * sizeof(unsigned long long)));
@@ -2797,13 +3170,28 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
       // network (SCREEN_RATES) or straight to the FP64 integrator (SCREEN_ONLY)
       sa.screen_out = screen == NRG_SUR_SCREEN_RATES ? h->sur_cand : h->sur_list;
       sa.screen_slot = screen == NRG_SUR_SCREEN_RATES ? SUR_CAND : SUR_FALLBACK;
-      hipLaunchKernelGGL(rate_screen_kernel, dim3((unsigned)((n + RCHECK_CELLS - 1) / RCHECK_CELLS)),
+      sa.screen_fp64 = gate->screen_fp64 != 0;
+      sa.recheck = h->sur_recheck;
+      sa.recheck_n = h->sur_acc + SUR_RECHECK;
+      if (!h->recheck_blocks) {
+        // resident blocks of the FP64 pass, which strides over its list
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rate_screen_kernel, 256, 0) != hipSuccess ||
+            per_cu < 1)
+          per_cu = 1;
+        h->recheck_blocks = (long long)per_cu * h->num_cu;
+      }
+      // the FP32 pass over every cell, then the FP64 estimate on the cells it
+      // lists (a grid that reads their number on the device)
+      hipLaunchKernelGGL(screen_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, sa);
+      hipLaunchKernelGGL(rate_screen_kernel,
+                         dim3((unsigned)std::min<long long>(h->recheck_blocks, (n + RCHECK_CELLS - 1) / RCHECK_CELLS)),
                          dim3(256), 0, h->stream, sa);
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(sur, h->sur_acc, sizeof(sur), hipMemcpyDeviceToHost, h->stream));
       HIP_TRY(hipStreamSynchronize(h->stream));
       ncand = screen == NRG_SUR_SCREEN_RATES ? (long long)sur[SUR_CAND] : 0;
-      if (ncand > n || (long long)sur[SUR_FALLBACK] > n)
+      if (ncand > n || (long long)sur[SUR_FALLBACK] > n || (long long)sur[SUR_RECHECK] > n)
         return set_error(NRG_CHEM_E_HIP, "screen counts exceed %lld cells", n);
       sa.cand = h->sur_cand;
       sa.npos = ncand;
@@ -2873,6 +3261,7 @@ int nrg_chem_surrogate_step_dev(nrg_chem_t *h, nrg_mlp_t *mlp, const nrg_surroga
     stats->cells_frozen = (int64_t)sur[SUR_FROZEN];
     stats->cells_network = ncand;
     stats->screen_ms = ms_scr;
+    stats->cells_screen_fp64 = (int64_t)sur[SUR_RECHECK];
     stats->infer_ms = ms_inf;
     stats->gate_ms = ms_gate;
     stats->fallback_ms = ms_fb;

@@ -61,7 +61,7 @@ def test_struct_layouts():
     assert C.sizeof(capi.NrgChemStats) == 7 * 8 + 2 * 4 + 8 * 8 + 8
     assert C.sizeof(capi.NrgSurrogateNorm) == 4 * 16 * 4 + 4 + 7 * 4
     assert C.sizeof(capi.NrgSurrogateGate) == 13 * 8 + 4 + 7 * 4 + 4 * 32 * 8
-    assert C.sizeof(capi.NrgSurrogateStats) == 10 * 8 + 3 * 8 + C.sizeof(capi.NrgChemStats) + 3 * 8
+    assert C.sizeof(capi.NrgSurrogateStats) == 10 * 8 + 3 * 8 + C.sizeof(capi.NrgChemStats) + 4 * 8
 
 
 def test_default_opts_are_reference_literals():

@@ -286,6 +286,47 @@ def test_rate_screen_without_network_is_the_freeze_test_plus_fp64(engine, oracle
     assert np.all(o["E_f_prod"][frozen] == 0)
 
 
+def _kernels_state(mech, n=48, seed=41):
+    import torch
+    st = inputs.hot_kernels_3d(mech, (n, n, n), seed=seed, device="cuda:0", n_kernels=6)
+    for k in ("T", "p", "rho"):
+        st[k] = st[k].reshape(-1)
+    st["Y"] = st["Y"].reshape(S, -1)
+    return st, torch.zeros(n ** 3, dtype=torch.int8, device="cuda:0")
+
+
+@pytest.mark.parametrize("field", ["ignition", "hot_kernels"])
+@pytest.mark.parametrize("screen", ["only", "rates"])
+@pytest.mark.parametrize("tol", [(1e-6, 0.01), (1e-8, 1e-4)])
+def test_screen_fp32_pass_keeps_the_fp64_decisions(engine, mech, zero_net, field, screen, tol):
+    """The screen's first pass (FP32 exponentials with a bound on its distance
+    from the FP64 estimate) decides only the cells whose side of each threshold
+    the bound settles, and leaves the others to the FP64 estimate: outputs,
+    statuses and counters equal those of the screen with the FP64 estimate
+    deciding every live cell, on an igniting mixture and on hot kernels in fresh
+    gas, at the bench's thresholds and at 100x tighter ones."""
+    st, bc = _state(mech, n=20000, seed=31) if field == "ignition" else _kernels_state(mech)
+    dt = 1e-7
+    ytol, ttol = tol
+    scr = capi.NRG_SUR_SCREEN_ONLY if screen == "only" else capi.NRG_SUR_SCREEN_RATES
+    net, norm = (None, None) if screen == "only" else (zero_net, _norm(np.zeros(S + 1)))
+    kw = dict(dY_freeze=ytol, dT_freeze=ttol, screen=scr, rate_tol=1e-5 if screen == "rates" else 0.0)
+    a = _run(engine, net, norm, _gate(mech, **kw), st, bc, dt)
+    b = _run(engine, net, norm, _gate(mech, screen_fp64=True, **kw), st, bc, dt)
+    sa, sb = a["stats"], b["stats"]
+    live = int(sa.cells_inner - sa.cells_gated)
+    _report("screen_fp32_pass", field=field, screen=screen, dY_freeze=ytol, live=live,
+            frozen=sa.cells_frozen, decided_fp64=sa.cells_screen_fp64,
+            fp64_share="%.5f" % (sa.cells_screen_fp64 / max(live, 1)))
+    assert sb.cells_screen_fp64 == live
+    assert sa.cells_screen_fp64 < live and 0 < sa.cells_frozen < live
+    for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("cells_accepted", "cells_fallback", "cells_gated", "cells_frozen", "cells_network",
+              "reject_rate"):
+        assert getattr(sa, k) == getattr(sb, k), k
+
+
 def test_fallback_takes_its_handles_integrator_options(engine, mech):
     """The learned step on a handle created with a looser eps and the analytic
     Jacobian: the screen freezes the same cells as on the reference handle (its
