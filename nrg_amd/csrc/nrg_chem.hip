@@ -1417,14 +1417,13 @@ NRG_DI bool screen_estimate(const MechCoef &mc, double Tin, const double (&C)[S]
   unsigned absent = 0;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    // one species at a time: its coefficients are read after the previous
-    // species' values are formed (not all species' coefficients at once)
-    asm volatile("" ::: "memory");
     ok = ok && C[s] >= 0.0;
     absent |= (C[s] == 0.0) ? 1u << s : 0u;
     lnC[s] = C[s] > 0.0 ? ln_m32(C[s]) : 0.0;
     H[s] = h_molar<EstimateMath>(mc, T, s);
     G[s] = s_molar<EstimateMath>(mc, T, s) * (1.0 / R_GAS_J) - (H[s] * (1.0 / R_GAS_J)) * rT;
+    // each species' values are formed here, in turn: left to the compiler,
+    // every species' coefficients were loaded at once (twice the registers)
     asm volatile("" : "+v"(G[s]), "+v"(lnC[s]), "+v"(H[s]));
   }
 #pragma unroll
@@ -1590,7 +1589,6 @@ __global__ void __launch_bounds__(256) screen_kernel(SurArgs a)
     for (int i = 0; i < SA; ++i) {
       const int s = kSp.full[i];
       const double est = ((acc[i] * a.dt) * mc.W[s]) * rrho;
-      asm volatile("" ::: "memory");   // as in screen_estimate
       const double b = (((bnd[i] * a.dt) * mc.W[s]) * rrho) * (1.0 + 0x1p-40);
       in = in && fabs(est) + b <= a.dY_freeze;
       out = out || fabs(est) - b > a.dY_freeze;
@@ -1601,11 +1599,10 @@ __global__ void __launch_bounds__(256) screen_kernel(SurArgs a)
       bde += fabs(e) * (b * rW[s]);
       const double ea = fabs(e) + 0x1p-10 * (fabs(mc.a[s][0][5]) + fabs(mc.a[s][1][5]) + fabs(H[s]));
       dabs += ea * ((fabs(est) + b) * rW[s]);
-      asm volatile("" : "+v"(de), "+v"(bde), "+v"(dabs));
+      asm volatile("" : "+v"(de), "+v"(bde), "+v"(dabs));   // in turn, as in screen_estimate
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      asm volatile("" ::: "memory");
       cv += (C[s] * rrho) * (cp_molar(mc, T, s) - R_GAS_J);
       asm volatile("" : "+v"(cv));
     }
