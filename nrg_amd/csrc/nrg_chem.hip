@@ -148,6 +148,16 @@ struct StepArgs {
 };
 
 // Inner cell (inner-loop order) at position base + t of the launch's queue.
+// The 64-bit form alone: the integrator kernel's one call (at a refill) keeps
+// its register allocation without the second path.
+NRG_DI long long cell_offset_wide(const GridIdx &g, long long q)
+{
+  const long long i = q % g.ix + g.ilo0;
+  const long long j = (q / g.ix) % g.iy + g.ilo1;
+  const long long kk = q / ((long long)g.ix * g.iy) + g.ilo2;
+  return (i - g.lo0) + (long long)g.nx * ((j - g.lo1) + (long long)g.ny * (kk - g.lo2));
+}
+
 NRG_DI long long cell_of(const StepArgs &a, long long t)
 {
   return a.list ? (long long)a.list[a.base + t] : a.base + t;
@@ -245,7 +255,7 @@ NRG_DI bool start_cell(const StepArgs &a, long long t, long long q, long long &o
 #pragma unroll
     for (int s = 0; s <= S; ++s) cs[s] = src[rec_slot(REC_C, s, a.nchunk)];
   }
-  off = cell_offset(a.g, q);
+  off = cell_offset_wide(a.g, q);
   // a direct load's data is in LDS once vmcnt has counted it
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   if (REC_C_IN_LDS) {
