@@ -327,6 +327,31 @@ def test_screen_fp32_pass_keeps_the_fp64_decisions(engine, mech, zero_net, field
         assert getattr(sa, k) == getattr(sb, k), k
 
 
+def test_screen_defers_a_cell_on_the_threshold(engine, oracle, mech):
+    """A cell whose explicit estimate sits on dY_freeze (the oracle's value of
+    it) is one the FP32 pass cannot place on either side: it goes to the FP64
+    estimate, and outputs, statuses and counters equal those of the screen that
+    sends every cell there.  Three cells, one at a time."""
+    st, bc = _state(mech, seed=37)
+    dt = 1e-7
+    est, _ = _explicit_estimate(oracle, mech, st, dt)
+    live = (st["T"].cpu().numpy() >= 305.0) & (bc.cpu().numpy() == 0)
+    cand = np.flatnonzero(live & (est > 1e-10) & (est < 1e-3))
+    assert cand.size >= 3
+    for q0 in cand[np.linspace(0, cand.size - 1, 3).astype(int)]:
+        kw = dict(dY_freeze=float(est[q0]), dT_freeze=1e9, screen=capi.NRG_SUR_SCREEN_ONLY)
+        a = _run(engine, None, None, _gate(mech, **kw), st, bc, dt)
+        b = _run(engine, None, None, _gate(mech, screen_fp64=True, **kw), st, bc, dt)
+        sa, sb = a["stats"], b["stats"]
+        _report("screen_defers", cell=int(q0), dY_freeze="%.6e" % est[q0], decided_fp64=sa.cells_screen_fp64,
+                frozen=sa.cells_frozen)
+        assert 1 <= sa.cells_screen_fp64 < int(live.sum()) == sb.cells_screen_fp64
+        for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+            assert np.array_equal(a[k], b[k]), k
+        for k in ("cells_accepted", "cells_fallback", "cells_gated", "cells_frozen"):
+            assert getattr(sa, k) == getattr(sb, k), k
+
+
 def test_fallback_takes_its_handles_integrator_options(engine, mech):
     """The learned step on a handle created with a looser eps and the analytic
     Jacobian: the screen freezes the same cells as on the reference handle (its
