@@ -168,3 +168,65 @@ def test_task_setup_naming_zhang_runs_on_its_build(zhang, tmp_path):
     finally:
         a.close()
         b.close()
+
+
+def _screen_only(eng, cu, dt, S, ytol, ttol, fp64):
+    """One learned step with gate.screen = ONLY (no network) on eng."""
+    import ctypes as C
+
+    import torch
+    n = cu["T"].shape[0]
+    g = capi.NrgSurrogateGate()
+    g.screen = capi.NRG_SUR_SCREEN_ONLY
+    g.dY_freeze, g.dT_freeze = ytol, ttol
+    g.screen_fp64 = 1 if fp64 else 0
+    g.n_elements = 0
+    Yp = torch.zeros((S, n), dtype=torch.float64, device="cuda:0")
+    Yp2 = torch.full((S, n), 7.0, dtype=torch.float64, device="cuda:0")
+    Ef = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    status = torch.full((n,), -1, dtype=torch.int32, device="cuda:0")
+    Y = cu["Y"].contiguous()
+    stats = capi.NrgSurrogateStats()
+    torch.cuda.synchronize()
+    rc = eng.lib.nrg_chem_surrogate_step_dev(
+        eng.h, None, None, C.byref(g), C.byref(capi.flat_grid(n)), dt,
+        cu["T"].data_ptr(), cu["p"].data_ptr(), cu["rho"].data_ptr(),
+        capi.ptr_array([Y[s].data_ptr() for s in range(S)]), None,
+        capi.ptr_array([Yp[s].data_ptr() for s in range(S)]),
+        capi.ptr_array([Yp2[s].data_ptr() for s in range(S)]), Ef.data_ptr(), status.data_ptr(),
+        C.byref(stats))
+    capi.check(eng.lib, rc)
+    return dict(Y_prod=Yp.cpu().numpy(), Y_prod2=Yp2.cpu().numpy(), E_f_prod=Ef.cpu().numpy(),
+                status=status.cpu().numpy(), stats=stats)
+
+
+@pytest.mark.parametrize("name", sorted(MECHS))
+def test_screen_fp32_pass_keeps_the_fp64_decisions(name):
+    """The explicit-rate screen on each mechanism's own build (third-body,
+    Troe and irreversible reactions; 9 to 17 species): the FP32 pass decides
+    only where its bound settles the decision, so outputs, statuses and
+    counters equal those of the screen with the FP64 estimate deciding every
+    live cell."""
+    from nrg_amd.solver import ChemistryStep
+    mech = M.load_mechanism(*MECHS[name])
+    S = mech.species_number
+    eng = ChemistryStep(mech, capi.default_opts())
+    try:
+        st = (_acetylene_front(mech, 128, 64, 4321) if name == "ACETYLENE"
+              else inputs.flame_front_2d(mech, 128, 64, seed=4321))
+        cu = {k: v.to("cuda:0") for k, v in st.items()}
+        for ytol, ttol in ((1e-6, 0.01), (1e-4, 1.0)):
+            a = _screen_only(eng, cu, 1e-7, S, ytol, ttol, False)
+            b = _screen_only(eng, cu, 1e-7, S, ytol, ttol, True)
+            sa, sb = a["stats"], b["stats"]
+            live = int(sa.cells_inner - sa.cells_gated)
+            print("[%s] screen dY_freeze %g: live %d frozen %d decided by the FP64 estimate %d"
+                  % (name, ytol, live, sa.cells_frozen, sa.cells_screen_fp64), flush=True)
+            assert sb.cells_screen_fp64 == live and sa.cells_screen_fp64 < live
+            assert 0 < sa.cells_frozen < live
+            for k in ("Y_prod", "Y_prod2", "E_f_prod", "status"):
+                assert np.array_equal(a[k], b[k]), k
+            for k in ("cells_accepted", "cells_fallback", "cells_gated", "cells_frozen"):
+                assert getattr(sa, k) == getattr(sb, k), k
+    finally:
+        eng.close()
