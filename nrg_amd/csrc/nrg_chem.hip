@@ -16,6 +16,9 @@
 // NRG_PART splits a build into two objects: 1 = everything but the
 // integrator kernel's body, 2 = the integrator kernel alone (so that it can be
 // compiled with scheduling options of its own); 0 (default) = both in one.
+// With NRG_PART 2, NRG_BDF_JAC selects the kernel's instantiations: 1 = those
+// with the finite-difference Jacobian, 2 = those with the analytic one, unset
+// = all four.
 #ifndef NRG_PART
 #define NRG_PART 0
 #endif
@@ -528,12 +531,16 @@ extern template __global__ void chem_bdf_kernel<true, true>(StepArgs);
 #endif
 
 #if NRG_PART == 2
-// The integrator part ends here: its four kernels, compiled with their own
+// The integrator part ends here: its kernels, compiled with their own
 // scheduling options (the Makefile), and nothing else.
+#if !defined(NRG_BDF_JAC) || NRG_BDF_JAC == 1
 template __global__ void chem_bdf_kernel<false, false>(StepArgs);
-template __global__ void chem_bdf_kernel<false, true>(StepArgs);
 template __global__ void chem_bdf_kernel<true, false>(StepArgs);
+#endif
+#if !defined(NRG_BDF_JAC) || NRG_BDF_JAC == 2
+template __global__ void chem_bdf_kernel<false, true>(StepArgs);
 template __global__ void chem_bdf_kernel<true, true>(StepArgs);
+#endif
 }  // namespace kern
 }  // namespace NRG_TOPO_NS
 #pragma GCC visibility pop
